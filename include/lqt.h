@@ -1,0 +1,241 @@
+/*
+ * lqt.h - C ABI of the MI355X (gfx950) TPORd monodomain solver.
+ *
+ * Drop-in boundary for the reference's hot path.  Each entry point names
+ * the reference interface it replaces (paths relative to the reference
+ * repository).  Conventions: opaque handles; every call returns an
+ * lqt_status (0 = LQT_OK) and leaves a message for lqt_last_error() on
+ * failure; buffers passed in are host memory owned by the caller; device
+ * memory is owned by the handle; one handle must not be used from two
+ * threads at once, distinct handles are independent; units are the
+ * reference's (mV, ms, mm, mm^2/ms, pA/pF, mM, fmol).
+ */
+#ifndef LQT_H
+#define LQT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int lqt_status;
+#define LQT_OK 0
+#define LQT_ERR_ARG 1      /* invalid argument or size                     */
+#define LQT_ERR_HIP 2      /* HIP runtime error (no GPU, launch failure)   */
+#define LQT_ERR_IO 3       /* file could not be read or written           */
+#define LQT_ERR_RANGE 4    /* non-finite membrane potential detected      */
+#define LQT_ERR_STATE 5    /* call out of order                           */
+
+typedef enum { LQT_FP64 = 0, LQT_FP32 = 1 } lqt_precision;
+
+/* enum CellType of SingleCell/Cell.cc:20 (values kept). */
+enum { LQT_LVEPI = 0, LQT_LVENDO = 1, LQT_EPI = 2, LQT_ENDO = 3, LQT_MCELL = 4, LQT_NONE = 255 };
+
+/* Runtime form of the reference's compile-time model flags
+ * (SingleCell/TPORd.cc:30-36,45; GPU3D.cu:18-22). */
+typedef struct {
+    double iks_scale;  /* LQT1: GKs x iks_scale (TPORd.cc:705-707 uses 0.05)   */
+    int iso;           /* ISO1 beta-adrenergic flag (TPORd.cc:583,589,629,731) */
+    double pca_iso;    /* PCa multiplier under ISO1 (1.6 TPORd.cc, 4 GPU3D.cu) */
+    int suprep;        /* SUPREP gene therapy (TPORd.cc:709-728)               */
+    double dose_fmol;  /* SUPREP dose in fmol                                  */
+    double sca;        /* GKs base multiplier `sca` (TPORd.cc:45,696)          */
+    int clamp;         /* CLAMP: voltage held (TPORd.cc:929-933)               */
+    double v0;         /* initial V: -87.5 (TPORd.cc:144), -86.2 (GPU3D.cu:213) */
+} lqt_params;
+
+/* Named parameter sets reproducing each reference build:
+ * "cpu_default" (TPORd.cc shipped flags CON+ISO1), "gpu3d" (GPU3D.cu:18-22),
+ * "gpu2d" (new_11.cu:18-21), "config1" (CON, ISO off), "config3"
+ * (IKs x0.2, ISO on, PCa x4). */
+lqt_status lqt_params_preset(const char *name, lqt_params *out);
+
+/* SUPREP dose -> (fraction of suppressed native GKs, fraction of the
+ * replacing GKs), GPU3D.cu:885-904 / fit_dose_dependent.py:245-248. */
+lqt_status lqt_suprep_fractions(double dose_fmol, double *frac_wt, double *frac_rep);
+
+const char *lqt_last_error(void);
+int lqt_device_count(void);
+lqt_status lqt_device_synchronize(void);
+/* Name of the GPU the library runs on (gcnArchName) into buf. */
+lqt_status lqt_device_name(char *buf, int len);
+
+/* ---------------------------------------------------------------------------
+ * Cells: n independent TPORd cells on the GPU, one lane per cell.
+ * Replaces the Cell/TPORd object interface (SingleCell/Cell.cc:22-42,
+ * SingleCell/TPORd.cc:137-376) used by SingleCell.cc and OneD.cc:
+ * setIstim / setDVgap_dt / update / getV / outputAllStates / readinAllStates.
+ */
+typedef struct lqt_cells lqt_cells;
+
+/* celltype[n] in enum CellType; params[nparams] with nparams 1 or n. */
+lqt_status lqt_cells_create(int n, const uint8_t *celltype, const lqt_params *params, int nparams,
+                            lqt_precision prec, double dt, lqt_cells **out);
+lqt_status lqt_cells_destroy(lqt_cells *c);
+/* nsteps calls of TPORd::update() per cell with fixed Istim / dVgap_dt
+ * (host arrays of n, or NULL for 0).  Replaces setIstim+setDVgap_dt+update. */
+lqt_status lqt_cells_step(lqt_cells *c, int nsteps, const double *istim, const double *dvgap);
+/* All 40 states per cell in the order of TPORd::outputAllStates
+ * (TPORd.cc:200-252), out[n*40]. */
+lqt_status lqt_cells_get_state(lqt_cells *c, double *out);
+lqt_status lqt_cells_set_state(lqt_cells *c, const double *in);
+lqt_status lqt_cells_get_v(lqt_cells *c, double *out);
+/* Text state file of TPORd::outputAllStates (%4.10f, one value per line)
+ * for cell i, and its reader TPORd::readinAllStates (TPORd.cc:254-353). */
+lqt_status lqt_cells_write_state_file(lqt_cells *c, int i, const char *path);
+lqt_status lqt_cells_read_state_file(lqt_cells *c, int i, const char *path);
+
+/* Single-cell pacing protocol of SingleCell.cc:38-287 for all n cells at once
+ * (each with its own parameter set): S1 pacing, then the statistics of the
+ * last beat.  stats[n*10] = rest, dV/dt max, overshoot, amplitude, APD20,
+ * APD25, APD50, APD75, APD80, APD90 (SingleCell.cc:220-232).  If vtrace is
+ * not NULL, V of cell `trace_cell` every `trace_every` steps is written to
+ * vtrace[trace_cap]; *ntrace receives the count. */
+typedef struct {
+    double bcl, stim_start, stim_duration, stim_strength;
+    int nbeats;
+} lqt_pacing;
+lqt_status lqt_cells_pace(lqt_cells *c, const lqt_pacing *pace, double *stats,
+                          int trace_cell, int trace_every, double *vtrace, int64_t trace_cap,
+                          int64_t *ntrace);
+
+/* ---------------------------------------------------------------------------
+ * Tissue: monodomain 1D/2D/3D tissue with the reference's explicit update
+ * (GPU3D.cu:1724-1842 main loop, calc_du :370-385 / new_11.cu:352-417,
+ * Itot :425-1220, new_u_v :1222-1230).  Grid index (x*ny + y)*nz + z.
+ */
+typedef struct {
+    int nx, ny, nz;
+    double dx;              /* mm                                           */
+    const uint8_t *label;   /* nx*ny*nz enum CellType, LQT_NONE outside     */
+    const float *fibre;     /* nx*ny*nz*3 unit fibre vectors, or NULL       */
+    double d_fibre;         /* mm^2/ms along the fibre (D1 in new_11.cu)   */
+    double d_cross;         /* mm^2/ms across (D2)                         */
+} lqt_geometry;
+
+/* S1 train of GPU3D.cu:1757-1762 / set_Istim :397-409: site k receives
+ * `strength` while the in-cycle time t - floor(t/bcl)*bcl lies in
+ * [offset_k, offset_k + duration); optional S2 of set_S2Istim :411-422:
+ * `s2_strength` on the S2 sites while t in [s2_time, s2_time + s2_duration). */
+typedef struct {
+    double bcl, duration, strength;
+    double s2_time, s2_duration, s2_strength;  /* s2_duration 0 = no S2 */
+} lqt_stim_protocol;
+
+typedef struct lqt_tissue lqt_tissue;
+
+lqt_status lqt_tissue_create(const lqt_geometry *geo, const lqt_params *params, lqt_precision prec,
+                             double dt, lqt_tissue **out);
+lqt_status lqt_tissue_destroy(lqt_tissue *t);
+/* Reads the reference's voxel list "x y z fx fy fz label" (GPU3D.cu:1276-1297,
+ * labels 1-20 mapped as GPU3D.cu:865-871) into a tissue of the given grid. */
+lqt_status lqt_tissue_create_from_file(const char *path, int nx, int ny, int nz, double dx,
+                                       double d_fibre, double d_cross, const lqt_params *params,
+                                       lqt_precision prec, double dt, lqt_tissue **out);
+lqt_status lqt_tissue_info(const lqt_tissue *t, int64_t *nvox, int64_t *nghost);
+/* Host only (no device needed): compaction and ghost mirror map of a label
+ * grid as built by lqt_tissue_create (GPU3D.cu:1301-1333, 1487-1604).
+ * g[nx*ny*nz] receives k+1 for tissue voxel k (x,y,z order), -(k+1) for a
+ * ghost cell mirroring voxel k, 0 elsewhere. */
+lqt_status lqt_geometry_ghost_map(int nx, int ny, int nz, const uint8_t *label, int32_t *g,
+                                  int64_t *nvox, int64_t *nghost);
+/* Stimulus sites are grid indices; offsets in ms (1000*t_act of the
+ * stimulation profile, GPU3D.cu:1691-1721). */
+lqt_status lqt_tissue_set_stimulus(lqt_tissue *t, const lqt_stim_protocol *p, int64_t nsites,
+                                   const int64_t *site_cells, const float *site_offset_ms,
+                                   int64_t ns2, const int64_t *s2_cells);
+/* Reads stimulationprofile.cnd ("x y z f f f t_act[s] s", GPU3D.cu:1691-1721). */
+lqt_status lqt_tissue_set_stimulus_file(lqt_tissue *t, const lqt_stim_protocol *p, const char *path);
+/* Advance nsteps steps of dt.  Returns LQT_ERR_RANGE if V became non-finite. */
+lqt_status lqt_tissue_step(lqt_tissue *t, int64_t nsteps);
+/* Same, on the caller's HIP stream (hipStream_t passed as void*). */
+lqt_status lqt_tissue_step_async(lqt_tissue *t, int64_t nsteps, void *stream);
+lqt_status lqt_tissue_time(const lqt_tissue *t, double *time_ms, int64_t *step);
+/* V of every tissue voxel in compacted (x,y,z) order, out[nvox]. */
+lqt_status lqt_tissue_read_v(lqt_tissue *t, double *out);
+/* All 40 states of tissue voxel k in the 40-line order, out[40]. */
+lqt_status lqt_tissue_read_state(lqt_tissue *t, int64_t k, double *out);
+lqt_status lqt_tissue_write_state(lqt_tissue *t, int64_t k, const double *in);
+/* Counts of V < -90 mV, V > 200 mV and non-finite V (GPU3D.cu:1796-1829),
+ * reduced on the device. */
+lqt_status lqt_tissue_v_guard(lqt_tissue *t, int64_t counts[3]);
+/* Legacy VTK STRUCTURED_POINTS of V on every `stride`-th voxel, -100 outside
+ * (GPU3D.cu:123-163 writes stride 3). */
+lqt_status lqt_tissue_write_vtk(lqt_tissue *t, const char *path, int stride);
+/* Binary checkpoint of time and full state; restores into a tissue of the
+ * same geometry. */
+lqt_status lqt_tissue_save(lqt_tissue *t, const char *path);
+lqt_status lqt_tissue_load(lqt_tissue *t, const char *path);
+/* Average device time of the fused step kernel over the last call of
+ * lqt_tissue_step, in microseconds (HIP events on the tissue's stream). */
+lqt_status lqt_tissue_last_kernel_us(const lqt_tissue *t, double *us_step_kernel, double *us_ghost_kernel);
+/* Algorithmic HBM bytes of one fused step kernel launch (per voxel state,
+ * V and static data, read and written once). */
+lqt_status lqt_tissue_bytes_per_step(const lqt_tissue *t, double *bytes);
+
+/* ---------------------------------------------------------------------------
+ * Strands: many 1D transmural strands (OneD.cc:40-448) in one persistent
+ * launch, each with its own length, cell types, parameters and pacing.
+ * Per strand the update is OneD.cc:315-398 (Jacobi snapshot of V, no-flux
+ * ends by copy, S1 on the first cells); the pseudo-ECG of
+ * PlotScript/plot1DECG_compare.py:82-105 is reduced on the device.
+ */
+typedef struct {
+    int ncell;            /* 2..512 cells                                   */
+    int cell_offset;      /* first entry in the celltype / coeff arrays     */
+    int param_index;      /* entry in the params array                      */
+    double dx;            /* mm                                             */
+    double bcl, stop_time;          /* ms; loop is time <= stop_time        */
+    double stim_start, stim_duration, stim_strength;
+    int nstim;            /* cells 0..nstim-1 are stimulated                */
+    int rec_every;        /* pseudo-ECG sample every rec_every steps        */
+    double rec_from;      /* ms; ECG recorded for time >= rec_from          */
+    double x0;            /* electrode position (mm from cell 0)            */
+    int measure_beat;     /* beat k = floor(t/bcl) in which act/APD/EAD are taken */
+} lqt_strand_desc;
+
+typedef struct {
+    double qt;                 /* ms, from the pseudo-ECG                   */
+    double apd90[3];           /* ms, mean over ENDO, MCELL, EPI cells      */
+    int ead;                   /* 1 if any cell shows an EAD or fails to repolarise */
+    double act_first, act_last;/* ms, earliest/latest -30 mV crossing      */
+    double cv;                 /* m/s between cells at 10% and 90% of length */
+    int64_t steps;
+} lqt_strand_result;
+
+typedef struct lqt_strands lqt_strands;
+
+/* celltype/coeff indexed by desc.cell_offset + i; coeff may be NULL (0.154). */
+lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc *desc, const uint8_t *celltype,
+                              const double *coeff, const lqt_params *params, int nparams,
+                              lqt_precision prec, double dt, lqt_strands **out);
+lqt_status lqt_strands_destroy(lqt_strands *s);
+/* Runs every strand to its stop time in one persistent launch (captured in
+ * a HIP graph on first use).  Resets the strands to the initial state first. */
+lqt_status lqt_strands_run(lqt_strands *s);
+lqt_status lqt_strands_results(lqt_strands *s, lqt_strand_result *out);
+/* Per-cell activation time, APD90 and EAD flag of strand i (ncell each). */
+lqt_status lqt_strands_cell_results(lqt_strands *s, int i, double *act, double *apd90, int *ead);
+/* Pseudo-ECG samples of strand i; *n receives the count (call with
+ * phi == NULL to query). */
+lqt_status lqt_strands_ecg(lqt_strands *s, int i, double *phi, double *t0, int64_t *n);
+/* Final 40-state vectors of strand i (ncell*40). */
+lqt_status lqt_strands_final_state(lqt_strands *s, int i, double *out);
+/* Device time of the last lqt_strands_run (ms, HIP events). */
+lqt_status lqt_strands_last_ms(const lqt_strands *s, double *ms);
+/* Work-group size chosen by the autotuner for this batch (64..512). */
+lqt_status lqt_strands_block(const lqt_strands *s, int *block);
+
+/* Host only: QT interval of a pseudo-ECG trace as lqt_strands_results
+ * computes it (the reference never defines QT): samples
+ * phi[j] at t0 + j*dts; Q at t_q; T peak = largest |phi| in
+ * [t_qrs_end, t_end); T end = first later sample with |phi| <= 5 % of the
+ * T peak.  *qt = T end - t_q, or -1 if there is no T end. */
+lqt_status lqt_ecg_qt(const double *phi, int64_t n, double t0, double dts, double t_q,
+                      double t_qrs_end, double t_end, double *qt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

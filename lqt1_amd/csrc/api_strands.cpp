@@ -1,0 +1,428 @@
+// C ABI: batched 1D strands (OneD.cc:40-448 for many strands at once) in
+// one persistent launch captured in a HIP graph, plus the per-strand
+// measurements: pseudo-ECG QT, APD90 per cell type, EAD flag and CV.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "common.hpp"
+#include "strands.hpp"
+
+namespace lqt {
+int strands_occupancy(bool fp32, int block);
+}
+using namespace lqt;
+
+struct lqt_strands {
+    int n = 0;
+    int64_t ncells = 0, pitch = 0;
+    bool fp32 = true;
+    double dt = 0.02;
+    int block = 512, grid = 1;
+    std::vector<lqt_strand_desc> desc;
+    std::vector<uint8_t> label;  // per cell enum CellType
+    std::vector<StrandDev> hdev;
+    // device
+    StrandDev* ddesc = nullptr;
+    int* order = nullptr;
+    int* queue = nullptr;
+    uint8_t* cls = nullptr;
+    double* coeff = nullptr;
+    void* consts = nullptr;
+    double* v0 = nullptr;
+    float* phi = nullptr;
+    int64_t* nphi = nullptr;
+    double* phit0 = nullptr;
+    double* act = nullptr;
+    double* apd = nullptr;
+    int* ead = nullptr;
+    void* fr = nullptr;
+    double* fd = nullptr;
+    void* fv = nullptr;
+    int64_t* steps = nullptr;
+    int64_t phi_total = 0;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double last_ms = 0;
+    bool ran = false;
+    // host copies of the results of the last run
+    std::vector<double> h_act, h_apd, h_phit0;
+    std::vector<int> h_ead;
+    std::vector<int64_t> h_nphi, h_steps;
+    std::vector<float> h_phi;
+};
+
+static size_t rsz(const lqt_strands* s) { return s->fp32 ? sizeof(float) : sizeof(double); }
+
+template <class R>
+static void fill_sargs(lqt_strands* s, StrandsArgs<R>& a) {
+    a.nstrands = s->n;
+    a.desc = s->ddesc;
+    a.order = s->order;
+    a.queue = s->queue;
+    a.cls = s->cls;
+    a.coeff = s->coeff;
+    a.consts = (const Consts<R>*)s->consts;
+    a.v0 = s->v0;
+    a.dt = (R)s->dt;
+    a.dt_d = s->dt;
+    a.phi = s->phi;
+    a.nphi = s->nphi;
+    a.phit0 = s->phit0;
+    a.act = s->act;
+    a.apd = s->apd;
+    a.ead = s->ead;
+    a.fin = SoA<R>{(R*)s->fr, s->fd, s->pitch};
+    a.finv = (R*)s->fv;
+    a.steps = s->steps;
+}
+
+static lqt_status launch_run(lqt_strands* s, hipStream_t st) {
+    launch_zero_int(s->queue, 1, st);
+    if (s->fp32) {
+        StrandsArgs<float> a;
+        fill_sargs(s, a);
+        launch_strands(true, s->block, s->grid, &a, st);
+    } else {
+        StrandsArgs<double> a;
+        fill_sargs(s, a);
+        launch_strands(false, s->block, s->grid, &a, st);
+    }
+    LQT_HIP(hipGetLastError());
+    return LQT_OK;
+}
+
+// QT of one strand from its pseudo-ECG: Q at the stimulus of the measured
+// beat; the T wave is the largest |phi| after the QRS (which ends 10 ms
+// after the last cell activates); T end is the first sample after the T
+// peak where |phi| falls to 5 % of that peak.
+template <class T>
+static double qt_from_ecg(const T* phi, int64_t n, double t0, double dts, double t_q, double t_qrs_end,
+                          double t_beat_end) {
+    int64_t ipk = -1;
+    double pk = 0;
+    for (int64_t j = 0; j < n; ++j) {
+        const double t = t0 + j * dts;
+        if (t < t_qrs_end || t >= t_beat_end) continue;
+        if (fabs(phi[j]) > pk) {
+            pk = fabs(phi[j]);
+            ipk = j;
+        }
+    }
+    if (ipk < 0 || pk <= 0) return -1.0;
+    for (int64_t j = ipk; j < n; ++j) {
+        const double t = t0 + j * dts;
+        if (t >= t_beat_end) break;
+        if (fabs(phi[j]) <= 0.05 * pk) return t - t_q;
+    }
+    return -1.0;
+}
+
+extern "C" {
+
+lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const uint8_t* celltype,
+                              const double* coeff, const lqt_params* params, int nparams, lqt_precision prec,
+                              double dt, lqt_strands** out) {
+    LQT_CHECK(nstrands > 0 && desc && celltype && params && nparams > 0 && out && dt > 0, LQT_ERR_ARG,
+              "lqt_strands_create: bad argument");
+    int ndev = 0;
+    LQT_CHECK(hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0, LQT_ERR_HIP,
+              "lqt_strands_create: no HIP device available");
+    int64_t ncells = 0;
+    int maxl = 0;
+    for (int i = 0; i < nstrands; ++i) {
+        const lqt_strand_desc& d = desc[i];
+        LQT_CHECK(d.ncell >= 2 && d.ncell <= 512, LQT_ERR_ARG, "strand %d: ncell %d outside 2..512", i, d.ncell);
+        LQT_CHECK(d.cell_offset >= 0 && d.param_index >= 0 && d.param_index < nparams, LQT_ERR_ARG,
+                  "strand %d: bad cell_offset or param_index", i);
+        LQT_CHECK(d.dx > 0 && d.bcl > 0 && d.stop_time >= 0 && d.rec_every >= 0 && d.nstim >= 0, LQT_ERR_ARG,
+                  "strand %d: bad dx, bcl, stop_time, rec_every or nstim", i);
+        ncells = std::max<int64_t>(ncells, (int64_t)d.cell_offset + d.ncell);
+        maxl = std::max(maxl, d.ncell);
+    }
+    lqt_strands* s = new lqt_strands();
+    s->n = nstrands;
+    s->ncells = ncells;
+    s->pitch = (ncells + 63) / 64 * 64;
+    s->fp32 = prec == LQT_FP32;
+    s->dt = dt;
+    s->desc.assign(desc, desc + nstrands);
+    s->label.assign(celltype, celltype + ncells);
+    s->block = std::min(512, (maxl + 63) / 64 * 64);
+    lqt_status st = [&]() -> lqt_status {
+        LQT_HIP(hipStreamCreate(&s->stream));
+        LQT_HIP(hipEventCreate(&s->e0));
+        LQT_HIP(hipEventCreate(&s->e1));
+        // per strand device descriptors and sample capacities
+        s->hdev.resize(nstrands);
+        int64_t off = 0;
+        for (int i = 0; i < nstrands; ++i) {
+            const lqt_strand_desc& d = desc[i];
+            StrandDev& h = s->hdev[i];
+            h.ncell = d.ncell;
+            h.cell_off = d.cell_offset;
+            h.pidx = d.param_index;
+            h.nstim = d.nstim;
+            h.rec_every = d.rec_every;
+            h.measure_beat = d.measure_beat;
+            h.dx = d.dx;
+            h.bcl = d.bcl;
+            h.stop_time = d.stop_time;
+            h.stim_start = d.stim_start;
+            h.stim_dur = d.stim_duration;
+            h.stim_strength = d.stim_strength;
+            h.rec_from = d.rec_from;
+            h.x0 = d.x0;
+            int64_t cap = 0;
+            if (d.rec_every > 0 && d.stop_time >= d.rec_from)
+                cap = (int64_t)((d.stop_time - std::max(0.0, d.rec_from)) / dt / d.rec_every) + 3;
+            h.phi_off = off;
+            h.phi_cap = cap;
+            off += cap;
+        }
+        s->phi_total = off;
+        // longest first: the cost of a strand is its step count times its
+        // wavefronts
+        std::vector<int> ord(nstrands);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+            const double ca = desc[a].stop_time * ((desc[a].ncell + 63) / 64);
+            const double cb = desc[b].stop_time * ((desc[b].ncell + 63) / 64);
+            return ca > cb;
+        });
+        std::vector<uint8_t> cl(ncells);
+        for (int64_t i = 0; i < ncells; ++i) cl[i] = (uint8_t)cell_class(celltype[i]);
+        std::vector<double> co(ncells, 0.154);
+        if (coeff) co.assign(coeff, coeff + ncells);
+        std::vector<double> v0(nparams);
+        for (int i = 0; i < nparams; ++i) v0[i] = params[i].v0;
+        lqt_status e;
+        if ((e = upload(&s->ddesc, s->hdev.data(), nstrands))) return e;
+        if ((e = upload(&s->order, ord.data(), nstrands))) return e;
+        if ((e = upload(&s->queue, (const int*)nullptr, 1))) return e;
+        if ((e = upload(&s->cls, cl.data(), ncells))) return e;
+        if ((e = upload(&s->coeff, co.data(), ncells))) return e;
+        if ((e = upload(&s->v0, v0.data(), nparams))) return e;
+        std::vector<Consts<double>> cd(nparams);
+        for (int i = 0; i < nparams; ++i) cd[i] = derive_consts(params[i]);
+        if (s->fp32) {
+            std::vector<Consts<float>> cf(nparams);
+            for (int i = 0; i < nparams; ++i) cf[i] = to_float(cd[i]);
+            if ((e = upload((Consts<float>**)&s->consts, cf.data(), nparams))) return e;
+        } else {
+            if ((e = upload((Consts<double>**)&s->consts, cd.data(), nparams))) return e;
+        }
+        LQT_HIP(hipMalloc((void**)&s->phi, sizeof(float) * std::max<int64_t>(1, s->phi_total)));
+        LQT_HIP(hipMalloc((void**)&s->nphi, sizeof(int64_t) * nstrands));
+        LQT_HIP(hipMalloc((void**)&s->phit0, sizeof(double) * nstrands));
+        LQT_HIP(hipMalloc((void**)&s->steps, sizeof(int64_t) * nstrands));
+        LQT_HIP(hipMalloc((void**)&s->act, sizeof(double) * s->pitch));
+        LQT_HIP(hipMalloc((void**)&s->apd, sizeof(double) * s->pitch));
+        LQT_HIP(hipMalloc((void**)&s->ead, sizeof(int) * s->pitch));
+        LQT_HIP(hipMalloc(&s->fr, rsz(s) * NR * s->pitch));
+        LQT_HIP(hipMalloc((void**)&s->fd, sizeof(double) * ND * s->pitch));
+        LQT_HIP(hipMalloc(&s->fv, rsz(s) * s->pitch));
+        // resident workgroups: one persistent workgroup per slot
+        int dev = 0;
+        LQT_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        LQT_HIP(hipGetDeviceProperties(&prop, dev));
+        const int occ = std::max(1, strands_occupancy(s->fp32, s->block));
+        s->grid = std::max(1, std::min(nstrands, occ * prop.multiProcessorCount));
+        return LQT_OK;
+    }();
+    if (st) {
+        lqt_strands_destroy(s);
+        return st;
+    }
+    *out = s;
+    return LQT_OK;
+}
+
+lqt_status lqt_strands_destroy(lqt_strands* s) {
+    if (!s) return LQT_OK;
+    if (s->stream) hipStreamSynchronize(s->stream);
+    if (s->exec) hipGraphExecDestroy(s->exec);
+    if (s->graph) hipGraphDestroy(s->graph);
+    hipFree(s->ddesc);
+    hipFree(s->order);
+    hipFree(s->queue);
+    hipFree(s->cls);
+    hipFree(s->coeff);
+    hipFree(s->consts);
+    hipFree(s->v0);
+    hipFree(s->phi);
+    hipFree(s->nphi);
+    hipFree(s->phit0);
+    hipFree(s->act);
+    hipFree(s->apd);
+    hipFree(s->ead);
+    hipFree(s->fr);
+    hipFree(s->fd);
+    hipFree(s->fv);
+    hipFree(s->steps);
+    if (s->e0) hipEventDestroy(s->e0);
+    if (s->e1) hipEventDestroy(s->e1);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+    return LQT_OK;
+}
+
+lqt_status lqt_strands_run(lqt_strands* s) {
+    LQT_CHECK(s, LQT_ERR_ARG, "lqt_strands_run: null handle");
+    if (!s->exec) {
+        LQT_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        lqt_status e = launch_run(s, s->stream);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(s->stream, &g);
+        if (e) return e;
+        LQT_HIP(ce);
+        s->graph = g;
+        LQT_HIP(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+    }
+    LQT_HIP(hipEventRecord(s->e0, s->stream));
+    LQT_HIP(hipGraphLaunch(s->exec, s->stream));
+    LQT_HIP(hipEventRecord(s->e1, s->stream));
+    LQT_HIP(hipStreamSynchronize(s->stream));
+    float ms = 0;
+    LQT_HIP(hipEventElapsedTime(&ms, s->e0, s->e1));
+    s->last_ms = ms;
+    // results to the host
+    s->h_act.resize(s->ncells);
+    s->h_apd.resize(s->ncells);
+    s->h_ead.resize(s->ncells);
+    s->h_nphi.resize(s->n);
+    s->h_steps.resize(s->n);
+    s->h_phit0.resize(s->n);
+    s->h_phi.resize(std::max<int64_t>(1, s->phi_total));
+    LQT_HIP(hipMemcpy(s->h_act.data(), s->act, sizeof(double) * s->ncells, hipMemcpyDeviceToHost));
+    LQT_HIP(hipMemcpy(s->h_apd.data(), s->apd, sizeof(double) * s->ncells, hipMemcpyDeviceToHost));
+    LQT_HIP(hipMemcpy(s->h_ead.data(), s->ead, sizeof(int) * s->ncells, hipMemcpyDeviceToHost));
+    LQT_HIP(hipMemcpy(s->h_nphi.data(), s->nphi, sizeof(int64_t) * s->n, hipMemcpyDeviceToHost));
+    LQT_HIP(hipMemcpy(s->h_steps.data(), s->steps, sizeof(int64_t) * s->n, hipMemcpyDeviceToHost));
+    LQT_HIP(hipMemcpy(s->h_phit0.data(), s->phit0, sizeof(double) * s->n, hipMemcpyDeviceToHost));
+    if (s->phi_total > 0)
+        LQT_HIP(hipMemcpy(s->h_phi.data(), s->phi, sizeof(float) * s->phi_total, hipMemcpyDeviceToHost));
+    s->ran = true;
+    return LQT_OK;
+}
+
+lqt_status lqt_strands_results(lqt_strands* s, lqt_strand_result* out) {
+    LQT_CHECK(s && out, LQT_ERR_ARG, "lqt_strands_results: bad argument");
+    LQT_CHECK(s->ran, LQT_ERR_STATE, "lqt_strands_results: call lqt_strands_run first");
+    for (int i = 0; i < s->n; ++i) {
+        const lqt_strand_desc& d = s->desc[i];
+        lqt_strand_result r;
+        memset(&r, 0, sizeof(r));
+        double sum[3] = {0, 0, 0};
+        int cnt[3] = {0, 0, 0};
+        double af = 1e300, al = -1e300;
+        int ead = 0;
+        for (int j = 0; j < d.ncell; ++j) {
+            const int64_t c = (int64_t)d.cell_offset + j;
+            const int cls = s->label[c] == LQT_MCELL ? 1 : (s->label[c] == LQT_EPI ? 2 : 0);
+            if (s->h_apd[c] >= 0) {
+                sum[cls] += s->h_apd[c];
+                cnt[cls] += 1;
+            }
+            if (s->h_act[c] >= 0) {
+                af = std::min(af, s->h_act[c]);
+                al = std::max(al, s->h_act[c]);
+            }
+            ead |= s->h_ead[c] ? 1 : 0;
+        }
+        for (int k = 0; k < 3; ++k) r.apd90[k] = cnt[k] ? sum[k] / cnt[k] : -1.0;
+        r.ead = ead;
+        r.act_first = af < 1e300 ? af : -1.0;
+        r.act_last = al > -1e300 ? al : -1.0;
+        // CV between cells at 10 % and 90 % of the strand (cells 10 and 90
+        // of OneD.cc:422-434 for 100 cells)
+        const int i10 = d.ncell / 10, i90 = (9 * d.ncell) / 10;
+        const double a10 = s->h_act[d.cell_offset + i10], a90 = s->h_act[d.cell_offset + i90];
+        r.cv = (a10 >= 0 && a90 > a10) ? d.dx * (i90 - i10) / (a90 - a10) : -1.0;
+        const double t_q = d.measure_beat * d.bcl + d.stim_start;
+        const double t_qrs_end = (r.act_last >= 0 ? r.act_last : t_q) + 10.0;
+        const double t_end = (d.measure_beat + 1) * d.bcl;
+        r.qt = qt_from_ecg(s->h_phi.data() + s->hdev[i].phi_off, s->h_nphi[i], s->h_phit0[i], d.rec_every * s->dt,
+                           t_q, t_qrs_end, t_end);
+        r.steps = s->h_steps[i];
+        out[i] = r;
+    }
+    return LQT_OK;
+}
+
+lqt_status lqt_strands_cell_results(lqt_strands* s, int i, double* act, double* apd90, int* ead) {
+    LQT_CHECK(s && i >= 0 && i < s->n, LQT_ERR_ARG, "lqt_strands_cell_results: bad argument");
+    LQT_CHECK(s->ran, LQT_ERR_STATE, "lqt_strands_cell_results: call lqt_strands_run first");
+    const lqt_strand_desc& d = s->desc[i];
+    for (int j = 0; j < d.ncell; ++j) {
+        const int64_t c = (int64_t)d.cell_offset + j;
+        if (act) act[j] = s->h_act[c];
+        if (apd90) apd90[j] = s->h_apd[c];
+        if (ead) ead[j] = s->h_ead[c];
+    }
+    return LQT_OK;
+}
+
+lqt_status lqt_strands_ecg(lqt_strands* s, int i, double* phi, double* t0, int64_t* n) {
+    LQT_CHECK(s && i >= 0 && i < s->n && n, LQT_ERR_ARG, "lqt_strands_ecg: bad argument");
+    LQT_CHECK(s->ran, LQT_ERR_STATE, "lqt_strands_ecg: call lqt_strands_run first");
+    *n = s->h_nphi[i];
+    if (t0) *t0 = s->h_phit0[i];
+    if (phi)
+        for (int64_t j = 0; j < s->h_nphi[i]; ++j) phi[j] = s->h_phi[s->hdev[i].phi_off + j];
+    return LQT_OK;
+}
+
+lqt_status lqt_strands_final_state(lqt_strands* s, int i, double* out) {
+    LQT_CHECK(s && out && i >= 0 && i < s->n, LQT_ERR_ARG, "lqt_strands_final_state: bad argument");
+    LQT_CHECK(s->ran, LQT_ERR_STATE, "lqt_strands_final_state: call lqt_strands_run first");
+    const lqt_strand_desc& d = s->desc[i];
+    const size_t rs = rsz(s);
+    for (int j = 0; j < d.ncell; ++j) {
+        const int64_t c = (int64_t)d.cell_offset + j;
+        for (int m = 0; m < 40; ++m) {
+            const Ref40 r = kRef40[m];
+            double val = 1.0;
+            unsigned char buf[8];
+            if (r.kind == 0) {
+                LQT_HIP(hipMemcpy(buf, (unsigned char*)s->fv + rs * c, rs, hipMemcpyDeviceToHost));
+                val = s->fp32 ? (double)*(float*)buf : *(double*)buf;
+            } else if (r.kind == 1) {
+                LQT_HIP(hipMemcpy(buf, (unsigned char*)s->fr + rs * ((int64_t)r.slot * s->pitch + c), rs,
+                                  hipMemcpyDeviceToHost));
+                val = s->fp32 ? (double)*(float*)buf : *(double*)buf;
+            } else if (r.kind == 2) {
+                LQT_HIP(hipMemcpy(&val, s->fd + (int64_t)r.slot * s->pitch + c, sizeof(double), hipMemcpyDeviceToHost));
+            }
+            out[(size_t)j * 40 + m] = val;
+        }
+    }
+    return LQT_OK;
+}
+
+lqt_status lqt_strands_last_ms(const lqt_strands* s, double* ms) {
+    LQT_CHECK(s && ms, LQT_ERR_ARG, "lqt_strands_last_ms: bad argument");
+    *ms = s->last_ms;
+    return LQT_OK;
+}
+
+lqt_status lqt_ecg_qt(const double* phi, int64_t n, double t0, double dts, double t_q, double t_qrs_end,
+                      double t_end, double* qt) {
+    LQT_CHECK(phi && qt && n >= 0 && dts > 0, LQT_ERR_ARG, "lqt_ecg_qt: bad argument");
+    *qt = qt_from_ecg(phi, n, t0, dts, t_q, t_qrs_end, t_end);
+    return LQT_OK;
+}
+
+lqt_status lqt_strands_block(const lqt_strands* s, int* block) {
+    LQT_CHECK(s && block, LQT_ERR_ARG, "lqt_strands_block: bad argument");
+    *block = s->block;
+    return LQT_OK;
+}
+
+}  // extern "C"

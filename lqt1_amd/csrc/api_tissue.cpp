@@ -1,0 +1,780 @@
+// C ABI: the tissue handle.  Geometry set-up restates GPU3D.cu:1263-1335
+// (compaction), :1487-1604 (ghost mirror map) and new_11.cu:1496-1499 /
+// :381-407 (diffusion tensor and its one-sided divergence) in 3D; the time
+// loop is GPU3D.cu:1724-1842 with the stimulus of :1757-1762.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+using namespace lqt;
+
+struct lqt_tissue {
+    int nx = 0, ny = 0, nz = 0;
+    int px = 0, py = 0, pz = 0;  // padded dims
+    double dx = 0, dt = 0;
+    bool fp32 = true, aniso = false;
+    double d_fibre = 0, d_cross = 0;
+    int64_t nvox = 0, nghost = 0, ncell_pad = 0, pitch = 0;
+    std::vector<int32_t> h_vpos;   // padded index of voxel k
+    std::vector<int64_t> h_cell;   // unpadded grid index of voxel k
+    std::vector<int32_t> h_gmap;   // unpadded grid -> k+1 (0 outside)
+    std::vector<uint8_t> h_label;  // per voxel, enum CellType
+    lqt_params params{};
+    // device
+    int32_t* vpos = nullptr;
+    uint8_t* cls = nullptr;
+    void* grid[2] = {nullptr, nullptr};
+    int cur = 0;
+    void* r = nullptr;
+    double* d = nullptr;
+    void* fib = nullptr;
+    void* drift = nullptr;
+    int32_t* gdst = nullptr;
+    int32_t* gsrc = nullptr;
+    void* consts = nullptr;
+    float* stim_off = nullptr;
+    uint8_t* s2 = nullptr;
+    unsigned long long* guard = nullptr;
+    // stimulus
+    lqt_stim_protocol proto{};
+    int64_t nsites = 0, ns2 = 0;
+    double off_min = 0, off_max = 0;
+    // time
+    double time = 0;
+    int64_t step = 0;
+    hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> ev;
+    double us_step = 0, us_ghost = 0;
+};
+
+static size_t rsz(const lqt_tissue* t) { return t->fp32 ? sizeof(float) : sizeof(double); }
+
+// Ghost map of GPU3D.cu:1487-1604: every non-tissue cell with a tissue cell
+// among its 26 neighbours mirrors the tissue neighbour closest to the
+// outward normal estimated from the neighbour pattern.  Only tissue cells
+// count in the pattern and as candidates, so the result does not depend on
+// the sweep order.  The normal components are summed term by term in the
+// reference's order so that exact ties between candidates resolve the same
+// way.
+namespace {
+// neighbour n (1..26) of the reference numbering is at
+// (x + NX[n], y + NY[n], z + NZ[n])
+constexpr int NXo[27] = {0, -1, -1, -1, -1, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+constexpr int NYo[27] = {0, -1, -1, -1, 0, 0, 0, 1, 1, 1, -1, -1, -1, 0, 0, 1, 1, 1, -1, -1, -1, 0, 0, 0, 1, 1, 1};
+constexpr int NZo[27] = {0, -1, 0, 1, -1, 0, 1, -1, 0, 1, -1, 0, 1, -1, 1, -1, 0, 1, -1, 0, 1, -1, 0, 1, -1, 0, 1};
+// signed terms (+-n, weight class 1/2/3 = 1, 1/sqrt2, 1/sqrt3) of the z, y
+// and x components, in the order written at GPU3D.cu:1565-1571
+struct Term {
+    int8_t n, w;
+};
+constexpr Term IC[18] = {{3, 3},   {-1, 3},  {6, 2},  {9, 3},   {-7, 3},  {-4, 2},  {12, 2},  {-10, 2}, {14, 1},
+                         {17, 2},  {-15, 2}, {-13, 1}, {20, 3}, {-18, 3}, {23, 2},  {26, 3},  {-24, 3}, {-21, 2}};
+constexpr Term IR[18] = {{9, 3},   {-2, 2},  {-3, 3}, {-1, 3},  {8, 2},   {7, 3},   {17, 2},  {-11, 1}, {-12, 2},
+                         {-10, 2}, {16, 1},  {15, 2}, {26, 3},  {-19, 2}, {-20, 3}, {-18, 3}, {25, 2},  {24, 3}};
+constexpr Term IL[18] = {{18, 3}, {19, 2}, {20, 3}, {21, 2}, {22, 1}, {23, 2}, {24, 3}, {25, 2}, {26, 3},
+                         {-1, 3}, {-2, 2}, {-3, 3}, {-4, 2}, {-5, 1}, {-6, 2}, {-7, 3}, {-8, 2}, {-9, 3}};
+}  // namespace
+
+// g[c] = k+1 (tissue voxel k), -(k+1) (ghost mirroring voxel k), 0 otherwise.
+static void ghost_map(int nx, int ny, int nz, const uint8_t* label, std::vector<int32_t>& g, int64_t* nvox,
+                      int64_t* nghost) {
+    const int64_t SX = (int64_t)ny * nz, SY = nz, ncell = (int64_t)nx * SX;
+    g.assign(ncell, 0);
+    int64_t num = 0;
+    for (int64_t c = 0; c < ncell; ++c)
+        if (label[c] != LQT_NONE) g[c] = (int32_t)(++num);
+    const double wt[4] = {0.0, 1.0, sqrt(2.0), sqrt(3.0)};
+    auto tis = [&](int x, int y, int z) -> int {
+        if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) return 0;
+        return g[x * SX + y * SY + z] > 0;
+    };
+    int64_t ng = 0;
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y)
+            for (int z = 0; z < nz; ++z) {
+                if (g[x * SX + y * SY + z] != 0) continue;
+                int gg[27];
+                int nb = 0;
+                for (int n = 1; n <= 26; ++n) nb += (gg[n] = tis(x + NXo[n], y + NYo[n], z + NZo[n]));
+                if (!nb) continue;
+                auto sum = [&](const Term* T) {
+                    double a = 0.0;
+                    for (int i = 0; i < 18; ++i) {
+                        const int n = T[i].n > 0 ? T[i].n : -T[i].n;
+                        const double term = T[i].w == 1 ? (double)gg[n] : gg[n] / wt[T[i].w];
+                        a = T[i].n > 0 ? a + term : a - term;
+                    }
+                    return a;
+                };
+                const double ic = sum(IC), ir = sum(IR), il = sum(IL);
+                double imax = fabs(ic);
+                if (fabs(ir) > imax) imax = fabs(ir);
+                if (fabs(il) > imax) imax = fabs(il);
+                double nxv = il / imax, nyv = ir / imax, nzv = ic / imax;
+                if (imax - 0 < 0.0001) nxv = nyv = nzv = 0;
+                double best = 1000;
+                int bx = 0, by = 0, bz = 0;
+                for (int oz = -1; oz <= 1; ++oz)
+                    for (int oy = -1; oy <= 1; ++oy)
+                        for (int ox = -1; ox <= 1; ++ox) {
+                            const double dis = sqrt(1.0 * (ox - nxv) * (ox - nxv) + (oy - nyv) * (oy - nyv) +
+                                                    (oz - nzv) * (oz - nzv));
+                            if (tis(x + ox, y + oy, z + oz) && dis < best) {
+                                best = dis;
+                                bx = ox;
+                                by = oy;
+                                bz = oz;
+                            }
+                        }
+                g[x * SX + y * SY + z] = -g[(x + bx) * SX + (y + by) * SY + (z + bz)];
+                ++ng;
+            }
+    if (nvox) *nvox = num;
+    if (nghost) *nghost = ng;
+}
+
+// Padded destination / source grid indices of every ghost cell.
+static void build_ghosts(const lqt_tissue* t, const std::vector<int32_t>& g, std::vector<int32_t>& dst,
+                         std::vector<int32_t>& src) {
+    const int64_t ncell = (int64_t)t->nx * t->ny * t->nz;
+    for (int64_t c = 0; c < ncell; ++c) {
+        if (g[c] >= 0) continue;
+        const int x = (int)(c / ((int64_t)t->ny * t->nz)), y = (int)((c / t->nz) % t->ny), z = (int)(c % t->nz);
+        dst.push_back((int32_t)(((int64_t)(x + 1) * t->py + (y + 1)) * t->pz + (z + 1)));
+        src.push_back(t->h_vpos[-g[c] - 1]);
+    }
+}
+
+template <class R>
+static void fill_args(lqt_tissue* t, TissueArgs<R>& a) {
+    a.nvox = t->nvox;
+    a.vpos = t->vpos;
+    a.cls = t->cls;
+    a.vold = (const R*)t->grid[t->cur];
+    a.vnew = (R*)t->grid[t->cur ^ 1];
+    a.sx = t->py * t->pz;
+    a.sy = t->pz;
+    a.fib = (const R*)t->fib;
+    a.drift = (const R*)t->drift;
+    a.pitch = t->pitch;
+    a.dcross = (R)t->d_cross;
+    a.ddiff = (R)(t->d_fibre - t->d_cross);
+    a.inv_h2 = (R)(1.0 / (t->dx * t->dx));
+    a.inv_4h2 = (R)(1.0 / (4.0 * t->dx * t->dx));
+    a.inv_2h = (R)(1.0 / (2.0 * t->dx));
+    a.st = SoA<R>{(R*)t->r, t->d, t->pitch};
+    a.consts = (const Consts<R>*)t->consts;
+    a.dt = (R)t->dt;
+    a.stim_on = 0;
+    a.tc = 0;
+    a.stim_dur = t->proto.duration;
+    a.stim_strength = (R)t->proto.strength;
+    a.stim_off = t->stim_off;
+    a.s2_on = 0;
+    a.s2_strength = (R)t->proto.s2_strength;
+    a.s2 = t->s2;
+    a.guard = t->guard;
+}
+
+static lqt_status tissue_upload_state(lqt_tissue* t, const std::vector<double>& s40);
+
+extern "C" {
+
+lqt_status lqt_tissue_create(const lqt_geometry* geo, const lqt_params* params, lqt_precision prec, double dt,
+                             lqt_tissue** out) {
+    LQT_CHECK(geo && params && out && geo->label, LQT_ERR_ARG, "lqt_tissue_create: null argument");
+    LQT_CHECK(geo->nx > 0 && geo->ny > 0 && geo->nz > 0 && geo->dx > 0 && dt > 0, LQT_ERR_ARG,
+              "lqt_tissue_create: bad dimensions or step");
+    const int64_t ncell = (int64_t)geo->nx * geo->ny * geo->nz;
+    const int64_t npad = (int64_t)(geo->nx + 2) * (geo->ny + 2) * (geo->nz + 2);
+    LQT_CHECK(npad < (int64_t)INT32_MAX, LQT_ERR_ARG, "lqt_tissue_create: grid too large for 32-bit indices");
+    int ndev = 0;
+    LQT_CHECK(hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0, LQT_ERR_HIP,
+              "lqt_tissue_create: no HIP device available");
+    lqt_tissue* t = new lqt_tissue();
+    t->nx = geo->nx;
+    t->ny = geo->ny;
+    t->nz = geo->nz;
+    t->px = geo->nx + 2;
+    t->py = geo->ny + 2;
+    t->pz = geo->nz + 2;
+    t->dx = geo->dx;
+    t->dt = dt;
+    t->fp32 = prec == LQT_FP32;
+    t->d_fibre = geo->d_fibre;
+    t->d_cross = geo->d_cross;
+    t->aniso = geo->fibre != nullptr && geo->d_fibre != geo->d_cross;
+    t->params = *params;
+    t->ncell_pad = npad;
+    // compaction in (x, y, z) order (GPU3D.cu:1307-1330)
+    t->h_gmap.assign(ncell, 0);
+    for (int64_t c = 0; c < ncell; ++c) {
+        if (geo->label[c] != LQT_NONE) {
+            t->h_cell.push_back(c);
+            t->h_gmap[c] = (int32_t)t->h_cell.size();
+        }
+    }
+    t->nvox = (int64_t)t->h_cell.size();
+    if (t->nvox == 0) {
+        delete t;
+        return set_error(LQT_ERR_ARG, "lqt_tissue_create: geometry has no tissue voxel");
+    }
+    t->pitch = (t->nvox + 255) / 256 * 256;
+    const int64_t SX = (int64_t)t->ny * t->nz, SY = t->nz;
+    t->h_vpos.resize(t->nvox);
+    t->h_label.resize(t->nvox);
+    std::vector<uint8_t> cl(t->nvox);
+    for (int64_t k = 0; k < t->nvox; ++k) {
+        const int64_t c = t->h_cell[k];
+        const int x = (int)(c / SX), y = (int)((c / SY) % t->ny), z = (int)(c % t->nz);
+        t->h_vpos[k] = (int32_t)(((int64_t)(x + 1) * t->py + (y + 1)) * t->pz + (z + 1));
+        t->h_label[k] = geo->label[c];
+        cl[k] = (uint8_t)cell_class(geo->label[c]);
+    }
+    std::vector<int32_t> gsigned, gd, gs;
+    ghost_map(t->nx, t->ny, t->nz, geo->label, gsigned, nullptr, nullptr);
+    build_ghosts(t, gsigned, gd, gs);
+    gsigned.clear();
+    gsigned.shrink_to_fit();
+    t->nghost = (int64_t)gd.size();
+
+    // tensor divergence b_j = sum_i d_i D_ij with the one-sided rule of
+    // new_11.cu:381-407 (tissue neighbours only)
+    std::vector<double> fibre_h, drift_h;
+    if (t->aniso) {
+        const double DD = geo->d_fibre - geo->d_cross;
+        std::vector<double> D((size_t)t->nvox * 6);
+        fibre_h.resize((size_t)t->nvox * 3);
+        for (int64_t k = 0; k < t->nvox; ++k) {
+            const float* f = geo->fibre + 3 * t->h_cell[k];
+            const double f0 = f[0], f1 = f[1], f2 = f[2];
+            fibre_h[3 * k] = f0;
+            fibre_h[3 * k + 1] = f1;
+            fibre_h[3 * k + 2] = f2;
+            double* Dk = &D[6 * k];
+            Dk[0] = geo->d_cross + DD * f0 * f0;
+            Dk[1] = geo->d_cross + DD * f1 * f1;
+            Dk[2] = geo->d_cross + DD * f2 * f2;
+            Dk[3] = DD * f0 * f1;
+            Dk[4] = DD * f0 * f2;
+            Dk[5] = DD * f1 * f2;
+        }
+        static const int comp[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+        const int64_t strides[3] = {SX, SY, 1};
+        const int dims[3] = {t->nx, t->ny, t->nz};
+        drift_h.assign((size_t)t->nvox * 3, 0.0);
+        for (int64_t k = 0; k < t->nvox; ++k) {
+            const int64_t c = t->h_cell[k];
+            const int co[3] = {(int)(c / SX), (int)((c / SY) % t->ny), (int)(c % t->nz)};
+            for (int i = 0; i < 3; ++i) {
+                const int32_t gp = co[i] + 1 < dims[i] ? t->h_gmap[c + strides[i]] : 0;
+                const int32_t gm = co[i] - 1 >= 0 ? t->h_gmap[c - strides[i]] : 0;
+                for (int j = 0; j < 3; ++j) {
+                    const int cj = comp[i][j];
+                    double dv = 0;
+                    if (gp > 0 && gm > 0) dv = (D[6 * (gp - 1) + cj] - D[6 * (gm - 1) + cj]) / (2 * t->dx);
+                    else if (gp > 0) dv = (D[6 * (gp - 1) + cj] - D[6 * k + cj]) / t->dx;
+                    else if (gm > 0) dv = (D[6 * k + cj] - D[6 * (gm - 1) + cj]) / t->dx;
+                    drift_h[3 * k + j] += dv;
+                }
+            }
+        }
+    }
+
+    lqt_status st = [&]() -> lqt_status {
+        LQT_HIP(hipStreamCreate(&t->stream));
+        const size_t rs = rsz(t);
+        lqt_status s2;
+        if ((s2 = upload(&t->vpos, t->h_vpos.data(), t->nvox))) return s2;
+        if ((s2 = upload(&t->cls, cl.data(), t->nvox))) return s2;
+        if ((s2 = upload(&t->gdst, gd.data(), gd.size()))) return s2;
+        if ((s2 = upload(&t->gsrc, gs.data(), gs.size()))) return s2;
+        LQT_HIP(hipMalloc(&t->grid[0], rs * npad));
+        LQT_HIP(hipMalloc(&t->grid[1], rs * npad));
+        LQT_HIP(hipMalloc(&t->r, rs * NR * t->pitch));
+        LQT_HIP(hipMalloc((void**)&t->d, sizeof(double) * ND * t->pitch));
+        LQT_HIP(hipMalloc((void**)&t->guard, sizeof(unsigned long long) * 3));
+        LQT_HIP(hipMemset(t->guard, 0, sizeof(unsigned long long) * 3));
+        if (t->aniso) {
+            // SoA: component j of voxel k at j*pitch + k
+            std::vector<unsigned char> fb(rs * 3 * t->pitch, 0), db(rs * 3 * t->pitch, 0);
+            for (int64_t k = 0; k < t->nvox; ++k)
+                for (int j = 0; j < 3; ++j) {
+                    const int64_t idx = (int64_t)j * t->pitch + k;
+                    if (t->fp32) {
+                        ((float*)fb.data())[idx] = (float)fibre_h[3 * k + j];
+                        ((float*)db.data())[idx] = (float)drift_h[3 * k + j];
+                    } else {
+                        ((double*)fb.data())[idx] = fibre_h[3 * k + j];
+                        ((double*)db.data())[idx] = drift_h[3 * k + j];
+                    }
+                }
+            if ((s2 = upload((unsigned char**)&t->fib, fb.data(), fb.size()))) return s2;
+            if ((s2 = upload((unsigned char**)&t->drift, db.data(), db.size()))) return s2;
+        }
+        const Consts<double> cd = derive_consts(*params);
+        if (t->fp32) {
+            const Consts<float> cf = to_float(cd);
+            if ((s2 = upload((Consts<float>**)&t->consts, &cf, 1))) return s2;
+        } else {
+            if ((s2 = upload((Consts<double>**)&t->consts, &cd, 1))) return s2;
+        }
+        std::vector<double> s40((size_t)t->nvox * 40);
+        for (int64_t k = 0; k < t->nvox; ++k) {
+            memcpy(&s40[(size_t)k * 40], kInit40, sizeof(kInit40));
+            s40[(size_t)k * 40] = params->v0;
+        }
+        return tissue_upload_state(t, s40);
+    }();
+    if (st) {
+        lqt_tissue_destroy(t);
+        return st;
+    }
+    *out = t;
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_destroy(lqt_tissue* t) {
+    if (!t) return LQT_OK;
+    if (t->stream) hipStreamSynchronize(t->stream);
+    hipFree(t->vpos);
+    hipFree(t->cls);
+    hipFree(t->grid[0]);
+    hipFree(t->grid[1]);
+    hipFree(t->r);
+    hipFree(t->d);
+    hipFree(t->fib);
+    hipFree(t->drift);
+    hipFree(t->gdst);
+    hipFree(t->gsrc);
+    hipFree(t->consts);
+    hipFree(t->stim_off);
+    hipFree(t->s2);
+    hipFree(t->guard);
+    for (hipEvent_t e : t->ev) hipEventDestroy(e);
+    if (t->stream) hipStreamDestroy(t->stream);
+    delete t;
+    return LQT_OK;
+}
+
+lqt_status lqt_geometry_ghost_map(int nx, int ny, int nz, const uint8_t* label, int32_t* g, int64_t* nvox,
+                                  int64_t* nghost) {
+    LQT_CHECK(nx > 0 && ny > 0 && nz > 0 && label && g, LQT_ERR_ARG, "lqt_geometry_ghost_map: bad argument");
+    std::vector<int32_t> m;
+    ghost_map(nx, ny, nz, label, m, nvox, nghost);
+    memcpy(g, m.data(), sizeof(int32_t) * m.size());
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_info(const lqt_tissue* t, int64_t* nvox, int64_t* nghost) {
+    LQT_CHECK(t, LQT_ERR_ARG, "lqt_tissue_info: null handle");
+    if (nvox) *nvox = t->nvox;
+    if (nghost) *nghost = t->nghost;
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_set_stimulus(lqt_tissue* t, const lqt_stim_protocol* p, int64_t nsites,
+                                   const int64_t* site_cells, const float* site_offset_ms, int64_t ns2,
+                                   const int64_t* s2_cells) {
+    LQT_CHECK(t && p && p->bcl > 0 && p->duration >= 0, LQT_ERR_ARG, "lqt_tissue_set_stimulus: bad protocol");
+    LQT_CHECK(nsites >= 0 && ns2 >= 0 && (nsites == 0 || site_cells) && (ns2 == 0 || s2_cells), LQT_ERR_ARG,
+              "lqt_tissue_set_stimulus: bad site arrays");
+    const int64_t ncell = (int64_t)t->nx * t->ny * t->nz;
+    std::vector<float> off(t->nvox, NAN);
+    std::vector<uint8_t> s2(t->nvox, 0);
+    double mn = 1e300, mx = -1e300;
+    int64_t used = 0;
+    for (int64_t i = 0; i < nsites; ++i) {
+        const int64_t c = site_cells[i];
+        LQT_CHECK(c >= 0 && c < ncell, LQT_ERR_ARG, "stimulus site %lld outside the grid", (long long)c);
+        const int32_t g = t->h_gmap[c];
+        if (g <= 0) continue;  // GPU3D.cu:1704: sites outside tissue are skipped
+        const float o = site_offset_ms ? site_offset_ms[i] : 0.0f;
+        off[g - 1] = o;
+        mn = std::min(mn, (double)o);
+        mx = std::max(mx, (double)o);
+        ++used;
+    }
+    int64_t used2 = 0;
+    for (int64_t i = 0; i < ns2; ++i) {
+        const int64_t c = s2_cells[i];
+        LQT_CHECK(c >= 0 && c < ncell, LQT_ERR_ARG, "S2 site %lld outside the grid", (long long)c);
+        const int32_t g = t->h_gmap[c];
+        if (g <= 0) continue;
+        s2[g - 1] = 1;
+        ++used2;
+    }
+    hipFree(t->stim_off);
+    hipFree(t->s2);
+    t->stim_off = nullptr;
+    t->s2 = nullptr;
+    lqt_status s;
+    if ((s = upload(&t->stim_off, off.data(), off.size()))) return s;
+    if ((s = upload(&t->s2, s2.data(), s2.size()))) return s;
+    t->proto = *p;
+    t->nsites = used;
+    t->ns2 = used2;
+    t->off_min = used ? mn : 0;
+    t->off_max = used ? mx : 0;
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_set_stimulus_file(lqt_tissue* t, const lqt_stim_protocol* p, const char* path) {
+    LQT_CHECK(t && p && path, LQT_ERR_ARG, "lqt_tissue_set_stimulus_file: null argument");
+    FILE* f = fopen(path, "r");
+    LQT_CHECK(f, LQT_ERR_IO, "cannot open %s", path);
+    std::vector<int64_t> cells;
+    std::vector<float> offs;
+    int x, y, z;
+    float a4, a5, a6, tact;
+    char tag[64];
+    // "x y z f f f t_act str" (GPU3D.cu:1700); t_act in seconds
+    while (fscanf(f, "%d %d %d %f %f %f %f %63s", &x, &y, &z, &a4, &a5, &a6, &tact, tag) >= 7) {
+        if (x < 0 || x >= t->nx || y < 0 || y >= t->ny || z < 0 || z >= t->nz) continue;
+        cells.push_back(((int64_t)x * t->ny + y) * t->nz + z);
+        offs.push_back(1000.0f * tact);
+    }
+    fclose(f);
+    return lqt_tissue_set_stimulus(t, p, (int64_t)cells.size(), cells.data(), offs.data(), 0, nullptr);
+}
+
+lqt_status lqt_tissue_step_async(lqt_tissue* t, int64_t nsteps, void* stream) {
+    LQT_CHECK(t && nsteps >= 0, LQT_ERR_ARG, "lqt_tissue_step: bad argument");
+    hipStream_t s = stream ? (hipStream_t)stream : t->stream;
+    const int64_t ntime = std::min<int64_t>(nsteps, 2048);
+    while ((int64_t)t->ev.size() < 4 * ntime) {
+        hipEvent_t e;
+        LQT_HIP(hipEventCreate(&e));
+        t->ev.push_back(e);
+    }
+    for (int64_t i = 0; i < nsteps; ++i) {
+        const bool timed = i < ntime;
+        if (t->fp32) {
+            TissueArgs<float> a;
+            fill_args(t, a);
+            const double tc = t->time - floor(t->time / t->proto.bcl) * t->proto.bcl;
+            a.stim_on = t->nsites > 0 && tc >= t->off_min && tc < t->off_max + t->proto.duration;
+            a.tc = tc;
+            a.s2_on = t->ns2 > 0 && t->proto.s2_duration > 0 && t->time >= t->proto.s2_time &&
+                      t->time < t->proto.s2_time + t->proto.s2_duration;
+            if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i], s));
+            launch_tissue_step(true, t->aniso, &a, t->nvox, s);
+        } else {
+            TissueArgs<double> a;
+            fill_args(t, a);
+            const double tc = t->time - floor(t->time / t->proto.bcl) * t->proto.bcl;
+            a.stim_on = t->nsites > 0 && tc >= t->off_min && tc < t->off_max + t->proto.duration;
+            a.tc = tc;
+            a.s2_on = t->ns2 > 0 && t->proto.s2_duration > 0 && t->time >= t->proto.s2_time &&
+                      t->time < t->proto.s2_time + t->proto.s2_duration;
+            if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i], s));
+            launch_tissue_step(false, t->aniso, &a, t->nvox, s);
+        }
+        if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i + 1], s));
+        LQT_HIP(hipGetLastError());
+        if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i + 2], s));
+        launch_ghost_fill(t->fp32, t->grid[t->cur ^ 1], t->gdst, t->gsrc, t->nghost, s);
+        if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i + 3], s));
+        LQT_HIP(hipGetLastError());
+        t->cur ^= 1;
+        t->time += t->dt;  // GPU3D.cu:1840 double accumulator
+        t->step += 1;
+    }
+    if (ntime > 0) {
+        LQT_HIP(hipStreamSynchronize(s));
+        double a = 0, b = 0;
+        for (int64_t i = 0; i < ntime; ++i) {
+            float m1 = 0, m2 = 0;
+            LQT_HIP(hipEventElapsedTime(&m1, t->ev[4 * i], t->ev[4 * i + 1]));
+            LQT_HIP(hipEventElapsedTime(&m2, t->ev[4 * i + 2], t->ev[4 * i + 3]));
+            a += m1;
+            b += m2;
+        }
+        t->us_step = 1000.0 * a / ntime;
+        t->us_ghost = 1000.0 * b / ntime;
+    }
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_step(lqt_tissue* t, int64_t nsteps) {
+    lqt_status s = lqt_tissue_step_async(t, nsteps, nullptr);
+    if (s) return s;
+    LQT_HIP(hipStreamSynchronize(t->stream));
+    unsigned long long g[3];
+    LQT_HIP(hipMemcpy(g, t->guard, sizeof(g), hipMemcpyDeviceToHost));
+    if (g[2]) return set_error(LQT_ERR_RANGE, "non-finite V in %llu voxel-steps (t = %.3f ms)", g[2], t->time);
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_time(const lqt_tissue* t, double* time_ms, int64_t* step) {
+    LQT_CHECK(t, LQT_ERR_ARG, "lqt_tissue_time: null handle");
+    if (time_ms) *time_ms = t->time;
+    if (step) *step = t->step;
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_read_v(lqt_tissue* t, double* out) {
+    LQT_CHECK(t && out, LQT_ERR_ARG, "lqt_tissue_read_v: bad argument");
+    LQT_HIP(hipStreamSynchronize(t->stream));
+    const size_t rs = rsz(t);
+    std::vector<unsigned char> g(rs * t->ncell_pad);
+    LQT_HIP(hipMemcpy(g.data(), t->grid[t->cur], g.size(), hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < t->nvox; ++k) {
+        const int64_t p = t->h_vpos[k];
+        out[k] = t->fp32 ? (double)((float*)g.data())[p] : ((double*)g.data())[p];
+    }
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_read_state(lqt_tissue* t, int64_t k, double* out) {
+    LQT_CHECK(t && out && k >= 0 && k < t->nvox, LQT_ERR_ARG, "lqt_tissue_read_state: bad argument");
+    LQT_HIP(hipStreamSynchronize(t->stream));
+    const size_t rs = rsz(t);
+    for (int j = 0; j < 40; ++j) {
+        const Ref40 m = kRef40[j];
+        double val = 1.0;
+        unsigned char buf[8];
+        if (m.kind == 0) {
+            LQT_HIP(hipMemcpy(buf, (unsigned char*)t->grid[t->cur] + rs * t->h_vpos[k], rs, hipMemcpyDeviceToHost));
+            val = t->fp32 ? (double)*(float*)buf : *(double*)buf;
+        } else if (m.kind == 1) {
+            LQT_HIP(hipMemcpy(buf, (unsigned char*)t->r + rs * ((int64_t)m.slot * t->pitch + k), rs,
+                              hipMemcpyDeviceToHost));
+            val = t->fp32 ? (double)*(float*)buf : *(double*)buf;
+        } else if (m.kind == 2) {
+            LQT_HIP(hipMemcpy(&val, t->d + (int64_t)m.slot * t->pitch + k, sizeof(double), hipMemcpyDeviceToHost));
+        }
+        out[j] = val;
+    }
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_write_state(lqt_tissue* t, int64_t k, const double* in) {
+    LQT_CHECK(t && in && k >= 0 && k < t->nvox, LQT_ERR_ARG, "lqt_tissue_write_state: bad argument");
+    LQT_HIP(hipStreamSynchronize(t->stream));
+    const size_t rs = rsz(t);
+    for (int j = 0; j < 40; ++j) {
+        const Ref40 m = kRef40[j];
+        unsigned char buf[8];
+        if (t->fp32) *(float*)buf = (float)in[j];
+        else *(double*)buf = in[j];
+        if (m.kind == 0) {
+            LQT_HIP(hipMemcpy((unsigned char*)t->grid[t->cur] + rs * t->h_vpos[k], buf, rs, hipMemcpyHostToDevice));
+        } else if (m.kind == 1) {
+            LQT_HIP(hipMemcpy((unsigned char*)t->r + rs * ((int64_t)m.slot * t->pitch + k), buf, rs,
+                              hipMemcpyHostToDevice));
+        } else if (m.kind == 2) {
+            LQT_HIP(hipMemcpy(t->d + (int64_t)m.slot * t->pitch + k, &in[j], sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    // refresh the ghost mirrors of the current grid
+    launch_ghost_fill(t->fp32, t->grid[t->cur], t->gdst, t->gsrc, t->nghost, t->stream);
+    LQT_HIP(hipGetLastError());
+    LQT_HIP(hipStreamSynchronize(t->stream));
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_v_guard(lqt_tissue* t, int64_t counts[3]) {
+    LQT_CHECK(t && counts, LQT_ERR_ARG, "lqt_tissue_v_guard: bad argument");
+    LQT_HIP(hipStreamSynchronize(t->stream));
+    unsigned long long g[3];
+    LQT_HIP(hipMemcpy(g, t->guard, sizeof(g), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 3; ++i) counts[i] = (int64_t)g[i];
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_last_kernel_us(const lqt_tissue* t, double* us_step_kernel, double* us_ghost_kernel) {
+    LQT_CHECK(t, LQT_ERR_ARG, "lqt_tissue_last_kernel_us: null handle");
+    if (us_step_kernel) *us_step_kernel = t->us_step;
+    if (us_ghost_kernel) *us_ghost_kernel = t->us_ghost;
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_bytes_per_step(const lqt_tissue* t, double* bytes) {
+    LQT_CHECK(t && bytes, LQT_ERR_ARG, "lqt_tissue_bytes_per_step: bad argument");
+    const double rs = (double)rsz(t);
+    // per voxel: V read + write, NR R-slots and ND doubles read + written,
+    // class byte and grid index, fibre + drift (anisotropic)
+    double per = 2.0 * (rs + NR * rs + ND * 8.0) + 1.0 + 4.0;
+    if (t->aniso) per += 6.0 * rs;
+    *bytes = per * (double)t->nvox;
+    return LQT_OK;
+}
+
+}  // extern "C"
+
+static lqt_status tissue_upload_state(lqt_tissue* t, const std::vector<double>& s40) {
+    const size_t rs = rsz(t);
+    std::vector<unsigned char> r(rs * NR * t->pitch, 0);
+    std::vector<double> d(ND * t->pitch, 0.0);
+    std::vector<unsigned char> g(rs * t->ncell_pad);
+    for (int64_t i = 0; i < t->ncell_pad; ++i) {
+        if (t->fp32) ((float*)g.data())[i] = -90.0f;
+        else ((double*)g.data())[i] = -90.0;
+    }
+    for (int64_t k = 0; k < t->nvox; ++k) {
+        for (int j = 0; j < 40; ++j) {
+            const Ref40 m = kRef40[j];
+            const double val = s40[(size_t)k * 40 + j];
+            if (m.kind == 0) {
+                if (t->fp32) ((float*)g.data())[t->h_vpos[k]] = (float)val;
+                else ((double*)g.data())[t->h_vpos[k]] = val;
+            } else if (m.kind == 1) {
+                const int64_t idx = (int64_t)m.slot * t->pitch + k;
+                if (t->fp32) ((float*)r.data())[idx] = (float)val;
+                else ((double*)r.data())[idx] = val;
+            } else if (m.kind == 2) {
+                d[(size_t)m.slot * t->pitch + k] = val;
+            }
+        }
+    }
+    LQT_HIP(hipMemcpy(t->r, r.data(), r.size(), hipMemcpyHostToDevice));
+    LQT_HIP(hipMemcpy(t->d, d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice));
+    LQT_HIP(hipMemcpy(t->grid[t->cur], g.data(), g.size(), hipMemcpyHostToDevice));
+    LQT_HIP(hipMemcpy(t->grid[t->cur ^ 1], g.data(), g.size(), hipMemcpyHostToDevice));
+    launch_ghost_fill(t->fp32, t->grid[t->cur], t->gdst, t->gsrc, t->nghost, t->stream);
+    launch_ghost_fill(t->fp32, t->grid[t->cur ^ 1], t->gdst, t->gsrc, t->nghost, t->stream);
+    LQT_HIP(hipGetLastError());
+    LQT_HIP(hipStreamSynchronize(t->stream));
+    return LQT_OK;
+}
+
+extern "C" {
+
+// Labels 1-20 of the reference's geometry files (GPU3D.cu:865-871 and the
+// label tests of Itot): 7-10 and 17-20 are epicardial, 4-6 and 14-16 are
+// mid-myocardial (M) cells, every other label is endocardial.
+static uint8_t label20_to_celltype(int l) {
+    if ((l >= 7 && l <= 10) || (l >= 17 && l <= 20)) return LQT_EPI;
+    if ((l >= 4 && l <= 6) || (l >= 14 && l <= 16)) return LQT_MCELL;
+    return LQT_ENDO;
+}
+
+lqt_status lqt_tissue_create_from_file(const char* path, int nx, int ny, int nz, double dx, double d_fibre,
+                                       double d_cross, const lqt_params* params, lqt_precision prec, double dt,
+                                       lqt_tissue** out) {
+    LQT_CHECK(path && nx > 0 && ny > 0 && nz > 0, LQT_ERR_ARG, "lqt_tissue_create_from_file: bad argument");
+    FILE* f = fopen(path, "r");
+    LQT_CHECK(f, LQT_ERR_IO, "cannot open %s", path);
+    const int64_t ncell = (int64_t)nx * ny * nz;
+    std::vector<uint8_t> label(ncell, LQT_NONE);
+    std::vector<float> fib((size_t)ncell * 3, 0.0f);
+    int x, y, z, l;
+    float f0, f1, f2;
+    int64_t nread = 0;
+    // one voxel per line: "x y z fx fy fz label" (GPU3D.cu:1276-1283)
+    while (fscanf(f, "%d %d %d %f %f %f %d", &x, &y, &z, &f0, &f1, &f2, &l) == 7) {
+        if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) {
+            fclose(f);
+            return set_error(LQT_ERR_ARG, "%s: voxel (%d,%d,%d) outside the %dx%dx%d grid", path, x, y, z, nx, ny, nz);
+        }
+        const int64_t c = ((int64_t)x * ny + y) * nz + z;
+        label[c] = label20_to_celltype(l);
+        fib[3 * c] = f0;
+        fib[3 * c + 1] = f1;
+        fib[3 * c + 2] = f2;
+        ++nread;
+    }
+    fclose(f);
+    LQT_CHECK(nread > 0, LQT_ERR_IO, "%s: no voxel read", path);
+    lqt_geometry g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.dx = dx;
+    g.label = label.data();
+    g.fibre = fib.data();
+    g.d_fibre = d_fibre;
+    g.d_cross = d_cross;
+    return lqt_tissue_create(&g, params, prec, dt, out);
+}
+
+lqt_status lqt_tissue_write_vtk(lqt_tissue* t, const char* path, int stride) {
+    LQT_CHECK(t && path && stride > 0, LQT_ERR_ARG, "lqt_tissue_write_vtk: bad argument");
+    std::vector<double> v(t->nvox);
+    lqt_status s = lqt_tissue_read_v(t, v.data());
+    if (s) return s;
+    FILE* f = fopen(path, "w");
+    LQT_CHECK(f, LQT_ERR_IO, "cannot open %s", path);
+    const int cx = (t->nx + stride - 1) / stride, cy = (t->ny + stride - 1) / stride, cz = (t->nz + stride - 1) / stride;
+    // legacy VTK STRUCTURED_POINTS as GPU3D.cu:123-163 writes it (x fastest,
+    // -100 for non-tissue); the DIMENSIONS line gives the counts written
+    fprintf(f, "# vtk DataFile Version 3.0\nvtk output\nASCII\nDATASET STRUCTURED_POINTS\n");
+    fprintf(f, "DIMENSIONS %d %d %d\nSPACING 1 1 1\nORIGIN 0 0 0\nPOINT_DATA %lld\n", cx, cy, cz,
+            (long long)cx * cy * cz);
+    fprintf(f, "SCALARS ImageFile float 1\nLOOKUP_TABLE default\n");
+    const int64_t SX = (int64_t)t->ny * t->nz, SY = t->nz;
+    for (int z = 0; z < t->nz; z += stride) {
+        for (int y = 0; y < t->ny; y += stride) {
+            for (int x = 0; x < t->nx; x += stride) {
+                const int32_t g = t->h_gmap[x * SX + y * SY + z];
+                if (g > 0) fprintf(f, "%g ", v[g - 1]);
+                else fprintf(f, "-100 ");
+            }
+            fprintf(f, "\n");
+        }
+        fprintf(f, "\n");
+    }
+    fclose(f);
+    return LQT_OK;
+}
+
+// Checkpoint: header, time, step, then every voxel's 40-state vector.
+static const char kMagic[8] = {'L', 'Q', 'T', 'T', 'I', 'S', '1', 0};
+
+lqt_status lqt_tissue_save(lqt_tissue* t, const char* path) {
+    LQT_CHECK(t && path, LQT_ERR_ARG, "lqt_tissue_save: bad argument");
+    LQT_HIP(hipStreamSynchronize(t->stream));
+    const size_t rs = rsz(t);
+    std::vector<unsigned char> r(rs * NR * t->pitch), g(rs * t->ncell_pad);
+    std::vector<double> d(ND * t->pitch);
+    LQT_HIP(hipMemcpy(r.data(), t->r, r.size(), hipMemcpyDeviceToHost));
+    LQT_HIP(hipMemcpy(d.data(), t->d, sizeof(double) * d.size(), hipMemcpyDeviceToHost));
+    LQT_HIP(hipMemcpy(g.data(), t->grid[t->cur], g.size(), hipMemcpyDeviceToHost));
+    FILE* f = fopen(path, "wb");
+    LQT_CHECK(f, LQT_ERR_IO, "cannot open %s", path);
+    const int64_t hdr[6] = {t->nx, t->ny, t->nz, t->nvox, (int64_t)t->fp32, t->step};
+    bool ok = fwrite(kMagic, 1, 8, f) == 8 && fwrite(hdr, sizeof(hdr), 1, f) == 1 &&
+              fwrite(&t->time, sizeof(double), 1, f) == 1 && fwrite(r.data(), 1, r.size(), f) == r.size() &&
+              fwrite(d.data(), sizeof(double), d.size(), f) == d.size() && fwrite(g.data(), 1, g.size(), f) == g.size();
+    fclose(f);
+    LQT_CHECK(ok, LQT_ERR_IO, "short write to %s", path);
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_load(lqt_tissue* t, const char* path) {
+    LQT_CHECK(t && path, LQT_ERR_ARG, "lqt_tissue_load: bad argument");
+    FILE* f = fopen(path, "rb");
+    LQT_CHECK(f, LQT_ERR_IO, "cannot open %s", path);
+    char magic[8];
+    int64_t hdr[6];
+    double time;
+    bool ok = fread(magic, 1, 8, f) == 8 && !memcmp(magic, kMagic, 8) && fread(hdr, sizeof(hdr), 1, f) == 1 &&
+              fread(&time, sizeof(double), 1, f) == 1;
+    if (!ok || hdr[0] != t->nx || hdr[1] != t->ny || hdr[2] != t->nz || hdr[3] != t->nvox || hdr[4] != (int64_t)t->fp32) {
+        fclose(f);
+        return set_error(LQT_ERR_IO, "%s: not a checkpoint of this geometry and precision", path);
+    }
+    const size_t rs = rsz(t);
+    std::vector<unsigned char> r(rs * NR * t->pitch), g(rs * t->ncell_pad);
+    std::vector<double> d(ND * t->pitch);
+    ok = fread(r.data(), 1, r.size(), f) == r.size() && fread(d.data(), sizeof(double), d.size(), f) == d.size() &&
+         fread(g.data(), 1, g.size(), f) == g.size();
+    fclose(f);
+    LQT_CHECK(ok, LQT_ERR_IO, "%s: truncated checkpoint", path);
+    LQT_HIP(hipStreamSynchronize(t->stream));
+    LQT_HIP(hipMemcpy(t->r, r.data(), r.size(), hipMemcpyHostToDevice));
+    LQT_HIP(hipMemcpy(t->d, d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice));
+    LQT_HIP(hipMemcpy(t->grid[t->cur], g.data(), g.size(), hipMemcpyHostToDevice));
+    LQT_HIP(hipMemcpy(t->grid[t->cur ^ 1], g.data(), g.size(), hipMemcpyHostToDevice));
+    t->time = time;
+    t->step = hdr[5];
+    return LQT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// Host-side declarations shared by the kernel translation units and the
+// C ABI implementation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "state.hpp"
+
+namespace lqt {
+
+// Per-cell statistics tracker of SingleCell.cc:144-215, kept in device
+// memory between launches.
+struct PaceTrack {
+    double oldV, rest, dvdt_max, overshoot, amplitude;
+    double repo[6];  // 20, 25, 50, 75, 80, 90 %
+    double apd[6];
+    int peakfound;
+    int pad;
+};
+
+struct PaceArgs {
+    double bcl, stim_start, stim_dur, stim_strength;
+    double numS1;       // nbeats as double (SingleCell.cc:45)
+    double dt;          // step of the double time accumulator
+    double time0;       // time at the first step of this launch
+    int64_t step0;      // index of the first step of this launch
+    int nsteps;
+    int trace_cell, trace_every;
+    int64_t trace_cap;
+    double* trace;      // device, or nullptr
+};
+
+template <class R>
+struct TissueArgs {
+    int64_t nvox;
+    const int32_t* vpos;       // padded grid index of voxel k
+    const uint8_t* cls;        // cell class of voxel k
+    const R* vold;             // dense padded V grid, step n
+    R* vnew;                   // dense padded V grid, step n+1
+    int32_t sx, sy;            // padded strides of x and y (z stride is 1)
+    const R* fib;              // SoA fibre components, 3 * pitch
+    const R* drift;            // SoA divergence of D, 3 * pitch
+    int64_t pitch;
+    R dcross, ddiff;           // D_cross, D_fibre - D_cross
+    R inv_h2, inv_4h2, inv_2h; // 1/dx^2, 1/(4dx^2), 1/(2dx)
+    SoA<R> st;
+    const Consts<R>* consts;
+    R dt;
+    // stimulus of this step
+    int stim_on;
+    double tc;                 // in-cycle time
+    double stim_dur;
+    R stim_strength;
+    const float* stim_off;
+    int s2_on;
+    R s2_strength;
+    const uint8_t* s2;
+    unsigned long long* guard; // counts: V < -90, V > 200, non-finite
+};
+
+// kernels_cells.hip
+void launch_cells_step(bool fp32, int n, void* r, double* d, int64_t pitch, void* v, const uint8_t* cls,
+                       const int* pidx, const void* consts, int nsteps, double dt, const double* ist,
+                       const double* dvgap, hipStream_t st);
+void launch_cells_pace(bool fp32, int n, void* r, double* d, int64_t pitch, void* v, const uint8_t* cls,
+                       const int* pidx, const void* consts, double dt, PaceTrack* track,
+                       const PaceArgs& a, hipStream_t st);
+
+// kernels_tissue.hip
+void launch_tissue_step(bool fp32, bool aniso, const void* args, int64_t nvox, hipStream_t st);
+void launch_ghost_fill(bool fp32, void* grid, const int32_t* dst, const int32_t* src, int64_t n,
+                       hipStream_t st);
+void launch_fill(bool fp32, void* p, int64_t n, double value, hipStream_t st);
+
+}  // namespace lqt

@@ -1,0 +1,229 @@
+// Persistent batched-strand kernel.  One workgroup owns one strand at a
+// time (cells on lanes, state in registers for the whole run), pulls the
+// next strand from a work counter when done, and advances its strand with
+// the loop of OneD.cc:315-436: Jacobi snapshot of V in LDS, no-flux ends
+// by copy, S1 on the first cells by the in-cycle time of the double time
+// accumulator, then the TPORd update.  The pseudo-ECG of
+// PlotScript/plot1DECG_compare.py:89-105 is reduced across the workgroup
+// every rec_every steps (wavefront shuffles, then one LDS slot per wave).
+#include "strands.hpp"
+
+namespace lqt {
+
+constexpr int MAXB = 512;
+
+template <class R>
+__device__ __forceinline__ R wave_sum(R x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+template <class R, int B>
+__global__ __launch_bounds__(B) void strands_kernel(StrandsArgs<R> A) {
+    __shared__ R vbuf[2][B + 2];
+    __shared__ float part[2][B / 64];
+    __shared__ int s_next;
+    const int tid = threadIdx.x;
+    const int wid = tid >> 6;
+
+    for (;;) {
+        if (tid == 0) s_next = atomicAdd(A.queue, 1);
+        __syncthreads();
+        const int q = s_next;
+        __syncthreads();
+        if (q >= A.nstrands) break;
+        const int sidx = A.order[q];
+        const StrandDev D = A.desc[sidx];
+        const int n = D.ncell;
+        const bool live = tid < n;
+        const int cell = D.cell_off + (live ? tid : 0);
+
+        // diffusion coefficient and its derivative (OneD.cc:108-131)
+        const double dxd = D.dx;
+        double Di = 0.0, dDi = 0.0;
+        if (live) {
+            Di = A.coeff[cell];
+            if (n == 1) dDi = 0.0;
+            else if (tid == 0) dDi = (A.coeff[cell + 1] - A.coeff[cell]) / dxd;
+            else if (tid == n - 1) dDi = (A.coeff[cell] - A.coeff[cell - 1]) / dxd;
+            else dDi = (A.coeff[cell + 1] - A.coeff[cell - 1]) / (2.0 * dxd);
+        }
+        const R RD = (R)Di, RdD = (R)dDi;
+        const R inv_dx = (R)(1.0 / dxd), inv_2dx = (R)(1.0 / (2.0 * dxd)), inv_dx2 = (R)(1.0 / (dxd * dxd));
+        // pseudo-ECG weight of this cell: one-sided at the ends
+        R wphi = R(0);
+        if (live) {
+            const double r = (tid * dxd - D.x0) * (tid * dxd - D.x0);
+            wphi = (R)((tid == 0 || tid == n - 1) ? 1.0 / r : 1.0 / (2.0 * r));
+        }
+        const int c = live ? A.cls[cell] : 0;
+        const Consts<R> cc = A.consts[D.pidx];
+
+        // TPORd::init (TPORd.cc:142-198)
+        CellState<R> s;
+        s.sm = 0; s.sh = 1; s.sj = 1; s.mL = 0; s.hL = 1; s.hLp = 1;
+        s.a = 0; s.iF = 1; s.iS = 1; s.ap = 0; s.iFp = 1; s.iSp = 1;
+        s.d = 0; s.ff = 1; s.fs = 1; s.fcaf = 1; s.fcas = 1; s.jca = 1; s.nca = 0; s.ffp = 1; s.fcafp = 1;
+        s.xrf = 0; s.xrs = 0; s.xs1 = 0; s.xs2 = 0; s.xk1 = 1; s.Jrelnp = 0; s.Jrelp = 0;
+        s.nass = 7; s.kss = 145; s.cai = (R)1.0e-4; s.cass = (R)1.0e-4;
+        s.nai = 7; s.ki = 145; s.cansr = 1.2; s.cajsr = 1.2; s.CaMKt = 0;
+        R v = (R)A.v0[D.pidx];
+
+        CellTrack tr;
+        tr.act = -1; tr.rep = -1; tr.vrest = 0; tr.vmax = -1e30; tr.vmin = 1e30; tr.flags = 0;
+
+        int64_t nrec = 0;
+        int pend = 0;          // a sample waits for its reduction
+        int fin_slot = -1;     // slot whose per-wave partials wait for thread 0
+        int64_t fin_idx = 0;
+        int64_t pend_idx = 0;
+        double time = 0.0;
+        int64_t step = 0;
+        int b = 0;
+        for (;; time += A.dt_d, ++step) {
+            const bool run = time <= D.stop_time;
+            // V^n into LDS with mirrored ends
+            if (live) {
+                vbuf[b][tid + 1] = v;
+                if (tid == 0) vbuf[b][0] = v;
+                if (tid == n - 1) vbuf[b][n + 1] = v;
+            }
+            __syncthreads();
+            // finish the sample whose per-wave partials were written last step
+            if (fin_slot >= 0 && tid == 0) {
+                float tot = 0.f;
+                for (int w = 0; w < B / 64; ++w) tot += part[fin_slot][w];
+                A.phi[D.phi_off + fin_idx] = tot;
+            }
+            fin_slot = -1;
+            // reduce the sample recorded at the end of the previous step
+            if (pend) {
+                const R dv = live ? vbuf[b][tid + 2] - vbuf[b][tid] : R(0);
+                const R tot = wave_sum<R>(-wphi * dv);
+                const int slot = (int)(pend_idx & 1);
+                if ((tid & 63) == 0) part[slot][wid] = (float)tot;
+                fin_slot = slot;
+                fin_idx = pend_idx;
+                pend = 0;
+            }
+            if (!run) {
+                // make the last partials visible, then finish them
+                __syncthreads();
+                if (fin_slot >= 0 && tid == 0) {
+                    float tot = 0.f;
+                    for (int w = 0; w < B / 64; ++w) tot += part[fin_slot][w];
+                    A.phi[D.phi_off + fin_idx] = tot;
+                }
+                break;
+            }
+            if (live) {
+                const R vl = vbuf[b][tid], vr = vbuf[b][tid + 2], vc = vbuf[b][tid + 1];
+                const R first = (tid == 0 || tid == n - 1) ? (vr - vl) * inv_dx : (vr - vl) * inv_2dx;
+                const R second = (vr + vl - R(2) * vc) * inv_dx2;
+                const R dvgap = RdD * first + RD * second;
+                const double tc = time - floor(time / D.bcl) * D.bcl;
+                const R ist = (tc >= D.stim_start && tc < D.stim_start + D.stim_dur && tid < D.nstim)
+                                  ? (R)D.stim_strength : R(0);
+                v = tpord_update<R, false>(s, v, c, cc, A.dt, ist, dvgap, nullptr);
+
+                // beat tracker, in the measured beat only (floor(t/BCL) == k)
+                if (floor(time / D.bcl) == D.measure_beat) {
+                    const double vf = (double)v;
+                    if (!(tr.flags & 1)) {
+                        tr.flags |= 1;
+                        tr.vrest = vf;
+                    }
+                    if (!(tr.flags & 2)) {
+                        if (vf >= -30.0) {
+                            tr.flags |= 2;
+                            tr.act = time;
+                            tr.vmax = vf;
+                        }
+                    } else if (!(tr.flags & 4)) {
+                        if (vf > tr.vmax) tr.vmax = vf;
+                        const double lvl = tr.vmax - 0.9 * (tr.vmax - tr.vrest);
+                        if (vf <= lvl) {
+                            tr.flags |= 4;
+                            tr.rep = time;
+                        } else if (time >= tr.act + 100.0) {
+                            // phase 2/3: a rise of more than 2 mV above the
+                            // lowest V since act+100 ms is an EAD
+                            if (vf < tr.vmin) tr.vmin = vf;
+                            if (vf > tr.vmin + 2.0) tr.flags |= 8;
+                        }
+                    }
+                }
+            }
+            if (D.rec_every > 0 && (step % D.rec_every) == 0 && time >= D.rec_from && nrec < D.phi_cap) {
+                if (nrec == 0 && tid == 0) A.phit0[sidx] = time;
+                pend = 1;
+                pend_idx = nrec++;
+            }
+            b ^= 1;
+        }
+
+        if (live) {
+            A.act[cell] = (tr.flags & 2) ? tr.act : -1.0;
+            A.apd[cell] = (tr.flags & 4) ? tr.rep - tr.act : -1.0;
+            int e = 0;
+            if (tr.flags & 8) e |= 1;
+            if ((tr.flags & 2) && !(tr.flags & 4)) e |= 2;
+            A.ead[cell] = e;
+            store_state(s, A.fin, cell);
+            A.finv[cell] = v;
+        }
+        if (tid == 0) {
+            A.nphi[sidx] = nrec;
+            A.steps[sidx] = step;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void zero_int_kernel(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+template <class R>
+static void launch_b(int block, int grid, const StrandsArgs<R>& a, hipStream_t st) {
+    switch (block) {
+        case 64: strands_kernel<R, 64><<<grid, 64, 0, st>>>(a); break;
+        case 128: strands_kernel<R, 128><<<grid, 128, 0, st>>>(a); break;
+        case 192: strands_kernel<R, 192><<<grid, 192, 0, st>>>(a); break;
+        case 256: strands_kernel<R, 256><<<grid, 256, 0, st>>>(a); break;
+        case 320: strands_kernel<R, 320><<<grid, 320, 0, st>>>(a); break;
+        case 384: strands_kernel<R, 384><<<grid, 384, 0, st>>>(a); break;
+        case 448: strands_kernel<R, 448><<<grid, 448, 0, st>>>(a); break;
+        default: strands_kernel<R, 512><<<grid, 512, 0, st>>>(a); break;
+    }
+}
+
+void launch_strands(bool fp32, int block, int grid, const void* args, hipStream_t st) {
+    if (fp32) launch_b<float>(block, grid, *(const StrandsArgs<float>*)args, st);
+    else launch_b<double>(block, grid, *(const StrandsArgs<double>*)args, st);
+}
+
+void launch_zero_int(int* p, int n, hipStream_t st) {
+    zero_int_kernel<<<(n + 255) / 256, 256, 0, st>>>(p, n);
+}
+
+// Resident workgroups per CU of the strand kernel for a block size.
+int strands_occupancy(bool fp32, int block) {
+    int nb = 0;
+    hipError_t e = hipSuccess;
+#define OCC(R, BB)                                                                                   \
+    case BB: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, strands_kernel<R, BB>, BB, 0); break;
+    if (fp32) {
+        switch (block) { OCC(float, 64) OCC(float, 128) OCC(float, 192) OCC(float, 256) OCC(float, 320)
+                         OCC(float, 384) OCC(float, 448) default: OCC(float, 512) }
+    } else {
+        switch (block) { OCC(double, 64) OCC(double, 128) OCC(double, 192) OCC(double, 256) OCC(double, 320)
+                         OCC(double, 384) OCC(double, 448) default: OCC(double, 512) }
+    }
+#undef OCC
+    return e == hipSuccess ? nb : 1;
+}
+
+}  // namespace lqt

@@ -1,0 +1,124 @@
+// Fused tissue step: diffusion term from V^n (the reference's calc_du,
+// GPU3D.cu:370-385 and new_11.cu:352-417 in 3D form), then the TPORd update
+// of the voxel with that term (Itot GPU3D.cu:425-1220 and new_u_v
+// :1222-1230), one voxel per lane over the compacted tissue list.
+//
+// V lives in a dense zero-padded grid (one cell of padding on every side),
+// so every neighbour read is a fixed offset from the voxel's grid index:
+// consecutive lanes read consecutive z, which keeps all 19 stencil reads
+// coalesced.  Ghost cells (the reference's mirror map, GPU3D.cu:1566-1604)
+// are refreshed after each step by ghost_fill_kernel; all other non-tissue
+// cells hold -90 mV (GPU3D.cu:305).
+#include "internal.hpp"
+
+namespace lqt {
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the MI355X.
+// This remap gives each XCD one contiguous range of tiles so that the
+// neighbouring z-rows and x/y-planes a tile touches are reused from that
+// XCD's own L2.
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
+    const int64_t per = nb >> 3, rem = nb & 7;
+    const int64_t x = b & 7, i = b >> 3;
+    return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
+template <class R, bool ANISO>
+__global__ __launch_bounds__(256) void tissue_step_kernel(TissueArgs<R> A) {
+    const int64_t k = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (k >= A.nvox) return;
+    const int64_t p = A.vpos[k];
+    const R* __restrict__ vo = A.vold;
+    const int64_t sx = A.sx, sy = A.sy;
+    const R vc = vo[p];
+    const R vxp = vo[p + sx], vxm = vo[p - sx];
+    const R vyp = vo[p + sy], vym = vo[p - sy];
+    const R vzp = vo[p + 1], vzm = vo[p - 1];
+    R du;
+    if (ANISO) {
+        const R vxx = (vxm + vxp - R(2) * vc) * A.inv_h2;
+        const R vyy = (vym + vyp - R(2) * vc) * A.inv_h2;
+        const R vzz = (vzm + vzp - R(2) * vc) * A.inv_h2;
+        const R vxy = (vo[p + sx + sy] + vo[p - sx - sy] - vo[p - sx + sy] - vo[p + sx - sy]) * A.inv_4h2;
+        const R vxz = (vo[p + sx + 1] + vo[p - sx - 1] - vo[p - sx + 1] - vo[p + sx - 1]) * A.inv_4h2;
+        const R vyz = (vo[p + sy + 1] + vo[p - sy - 1] - vo[p - sy + 1] - vo[p + sy - 1]) * A.inv_4h2;
+        const R f0 = A.fib[k], f1 = A.fib[A.pitch + k], f2 = A.fib[2 * A.pitch + k];
+        const R b0 = A.drift[k], b1 = A.drift[A.pitch + k], b2 = A.drift[2 * A.pitch + k];
+        const R dd = A.ddiff, dc = A.dcross;
+        // D = Dc*I + (Df-Dc) f f^T ; du = sum_ij D_ij d_i d_j V + sum_j b_j d_j V
+        du = (dc + dd * f0 * f0) * vxx + (dc + dd * f1 * f1) * vyy + (dc + dd * f2 * f2) * vzz +
+             R(2) * dd * (f0 * f1 * vxy + f0 * f2 * vxz + f1 * f2 * vyz) +
+             (b0 * (vxp - vxm) + b1 * (vyp - vym) + b2 * (vzp - vzm)) * A.inv_2h;
+    } else {
+        du = A.dcross * ((vxm + vxp - R(2) * vc) + (vym + vyp - R(2) * vc) + (vzm + vzp - R(2) * vc)) * A.inv_h2;
+    }
+
+    R ist = R(0);
+    if (A.stim_on) {
+        const float off = A.stim_off[k];
+        if (A.tc >= (double)off && A.tc < (double)off + A.stim_dur) ist = A.stim_strength;
+    }
+    if (A.s2_on && A.s2[k]) ist = A.s2_strength;
+
+    CellState<R> s;
+    load_state(s, A.st, k);
+    const int c = A.cls[k];
+    const R vn = tpord_update<R, false>(s, vc, c, *A.consts, A.dt, ist, du, nullptr);
+    store_state(s, A.st, k);
+    A.vnew[p] = vn;
+
+    // V guard of GPU3D.cu:1796-1829, one atomic per wavefront.
+    const bool lo = vn < R(-90), hi = vn > R(200), bad = !(vn == vn) || !(vn - vn == R(0));
+    const unsigned long long mlo = __ballot(lo), mhi = __ballot(hi), mbad = __ballot(bad);
+    if ((mlo | mhi | mbad) && (threadIdx.x & 63) == (__ffsll((long long)(mlo | mhi | mbad)) - 1)) {
+        if (mlo) atomicAdd(&A.guard[0], (unsigned long long)__popcll(mlo));
+        if (mhi) atomicAdd(&A.guard[1], (unsigned long long)__popcll(mhi));
+        if (mbad) atomicAdd(&A.guard[2], (unsigned long long)__popcll(mbad));
+    }
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void ghost_fill_kernel(R* __restrict__ g, const int32_t* __restrict__ dst,
+                                                          const int32_t* __restrict__ src, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) g[dst[i]] = g[src[i]];
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void fill_kernel(R* __restrict__ p, int64_t n, R value) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+void launch_tissue_step(bool fp32, bool aniso, const void* args, int64_t nvox, hipStream_t st) {
+    const int B = 256;
+    const int64_t G = (nvox + B - 1) / B;
+    if (fp32) {
+        const TissueArgs<float>& a = *(const TissueArgs<float>*)args;
+        if (aniso) tissue_step_kernel<float, true><<<G, B, 0, st>>>(a);
+        else tissue_step_kernel<float, false><<<G, B, 0, st>>>(a);
+    } else {
+        const TissueArgs<double>& a = *(const TissueArgs<double>*)args;
+        if (aniso) tissue_step_kernel<double, true><<<G, B, 0, st>>>(a);
+        else tissue_step_kernel<double, false><<<G, B, 0, st>>>(a);
+    }
+}
+
+void launch_ghost_fill(bool fp32, void* grid, const int32_t* dst, const int32_t* src, int64_t n,
+                       hipStream_t st) {
+    if (n == 0) return;
+    const int B = 256;
+    const int64_t G = (n + B - 1) / B;
+    if (fp32) ghost_fill_kernel<float><<<G, B, 0, st>>>((float*)grid, dst, src, n);
+    else ghost_fill_kernel<double><<<G, B, 0, st>>>((double*)grid, dst, src, n);
+}
+
+void launch_fill(bool fp32, void* p, int64_t n, double value, hipStream_t st) {
+    if (n == 0) return;
+    const int B = 256;
+    const int64_t G = (n + B - 1) / B;
+    if (fp32) fill_kernel<float><<<G, B, 0, st>>>((float*)p, n, (float)value);
+    else fill_kernel<double><<<G, B, 0, st>>>((double*)p, n, value);
+}
+
+}  // namespace lqt

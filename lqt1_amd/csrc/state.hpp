@@ -1,0 +1,89 @@
+// Structure-of-arrays storage of CellState and the host-side folding of the
+// run parameters into per-class constants.
+#pragma once
+
+#include <stdint.h>
+
+#include "tpord_model.hpp"
+
+// R-typed slots in storage order (the order of CellState).
+#define LQT_RFIELDS(X)                                                                              \
+    X(sm) X(sh) X(sj) X(mL) X(hL) X(hLp) X(a) X(iF) X(iS) X(ap) X(iFp) X(iSp) X(d) X(ff) X(fs)     \
+    X(fcaf) X(fcas) X(jca) X(nca) X(ffp) X(fcafp) X(xrf) X(xrs) X(xs1) X(xs2) X(xk1) X(Jrelnp)    \
+    X(Jrelp) X(nass) X(kss) X(cai) X(cass)
+#define LQT_DFIELDS(X) X(nai) X(ki) X(cansr) X(cajsr) X(CaMKt)
+
+namespace lqt {
+
+enum RSlot {
+#define LQT_ENUM(n) RS_##n,
+    LQT_RFIELDS(LQT_ENUM)
+#undef LQT_ENUM
+    RS_COUNT
+};
+enum DSlot {
+#define LQT_ENUM(n) DS_##n,
+    LQT_DFIELDS(LQT_ENUM)
+#undef LQT_ENUM
+    DS_COUNT
+};
+static_assert(RS_COUNT == NR, "R slot count");
+static_assert(DS_COUNT == ND, "D slot count");
+
+// SoA view: slot j of cell k is r[j*pitch + k] / d[j*pitch + k].
+template <class R>
+struct SoA {
+    R* r;
+    double* d;
+    int64_t pitch;
+};
+
+template <class R>
+__device__ __forceinline__ void load_state(CellState<R>& s, const SoA<R>& a, int64_t k) {
+#define LQT_LD(n) s.n = a.r[(int64_t)RS_##n * a.pitch + k];
+    LQT_RFIELDS(LQT_LD)
+#undef LQT_LD
+#define LQT_LD(n) s.n = a.d[(int64_t)DS_##n * a.pitch + k];
+    LQT_DFIELDS(LQT_LD)
+#undef LQT_LD
+}
+
+template <class R>
+__device__ __forceinline__ void store_state(const CellState<R>& s, const SoA<R>& a, int64_t k) {
+#define LQT_ST(n) a.r[(int64_t)RS_##n * a.pitch + k] = s.n;
+    LQT_RFIELDS(LQT_ST)
+#undef LQT_ST
+#define LQT_ST(n) a.d[(int64_t)DS_##n * a.pitch + k] = s.n;
+    LQT_DFIELDS(LQT_ST)
+#undef LQT_ST
+}
+
+// Location of each of the reference's 40 states (TPORd.cc:200-252 order)
+// in storage: kind 0 = V, 1 = R slot, 2 = D slot, 3 = constant 1 (the dead
+// hsp/jp gates, never updated by TPORd.cc).
+struct Ref40 {
+    int kind, slot;
+};
+static const Ref40 kRef40[40] = {
+    {0, 0},         {2, DS_nai},   {1, RS_nass},  {2, DS_ki},     {1, RS_kss},    {1, RS_cai},
+    {1, RS_cass},   {2, DS_cansr}, {2, DS_cajsr}, {1, RS_sm},     {1, RS_sh},     {1, RS_sj},
+    {3, 0},         {3, 0},        {1, RS_mL},    {1, RS_hL},     {1, RS_hLp},    {1, RS_a},
+    {1, RS_iF},     {1, RS_iS},    {1, RS_ap},    {1, RS_iFp},    {1, RS_iSp},    {1, RS_d},
+    {1, RS_ff},     {1, RS_fs},    {1, RS_fcaf},  {1, RS_fcas},   {1, RS_jca},    {1, RS_nca},
+    {1, RS_ffp},    {1, RS_fcafp}, {1, RS_xrf},   {1, RS_xrs},    {1, RS_xs1},    {1, RS_xs2},
+    {1, RS_xk1},    {1, RS_Jrelnp},{1, RS_Jrelp}, {2, DS_CaMKt},
+};
+
+// Initial values of TPORd::init (TPORd.cc:142-198) in the 40-line order.
+static const double kInit40[40] = {
+    -87.5, 7, 7, 145, 145, 1.0e-4, 1.0e-4, 1.2, 1.2, 0, 1, 1, 1, 1, 0, 1, 1, 0, 1, 1,
+    0,     1, 1, 0,   1,   1,      1,      1,   1,   0, 1, 1, 0, 0, 0, 0, 1, 0, 0, 0,
+};
+
+// Cell class of an enum CellType value: only EPI and MCELL are tested by
+// the model (TPORd.cc:523-1011); every other type behaves as ENDO.
+__host__ __device__ inline int cell_class(int celltype) {
+    return celltype == 2 ? CLS_EPI : (celltype == 4 ? CLS_M : CLS_ENDO);
+}
+
+}  // namespace lqt

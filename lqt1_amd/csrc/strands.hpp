@@ -1,0 +1,49 @@
+// Device-side description of a batch of 1D strands.
+#pragma once
+
+#include "internal.hpp"
+
+namespace lqt {
+
+struct StrandDev {
+    int ncell, cell_off, pidx, nstim;
+    int rec_every, measure_beat;
+    double dx, bcl, stop_time, stim_start, stim_dur, stim_strength, rec_from, x0;
+    int64_t phi_off, phi_cap;
+};
+
+// Per-cell beat tracker: -30 mV activation (OneD.cc:422), running peak,
+// 90 % repolarisation and the EAD test, all within beat `measure_beat`.
+struct CellTrack {
+    double act, rep;
+    double vrest, vmax, vmin;
+    int flags;  // bit0 entered beat, bit1 activated, bit2 repolarised, bit3 EAD
+};
+
+template <class R>
+struct StrandsArgs {
+    int nstrands;
+    const StrandDev* desc;
+    const int* order;          // strand processing order (longest first)
+    int* queue;                // work counter, zeroed before each run
+    const uint8_t* cls;        // per cell class
+    const double* coeff;       // per cell D (mm^2/ms)
+    const Consts<R>* consts;
+    const double* v0;          // per parameter set initial V
+    R dt;
+    double dt_d;
+    float* phi;                // pseudo-ECG samples
+    int64_t* nphi;             // per strand sample count
+    double* phit0;             // per strand time of the first sample
+    double* act;               // per cell
+    double* apd;               // per cell APD90
+    int* ead;                  // per cell flags: 1 EAD, 2 no repolarisation
+    SoA<R> fin;                // final state, pitch = total cells
+    R* finv;
+    int64_t* steps;            // per strand step count
+};
+
+void launch_strands(bool fp32, int block, int grid, const void* args, hipStream_t st);
+void launch_zero_int(int* p, int n, hipStream_t st);
+
+}  // namespace lqt

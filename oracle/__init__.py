@@ -45,6 +45,11 @@ def params(iks_scale=1.0, iso=1, pca_iso=1.6, suprep=0, dose_fmol=150.0, sca=1.0
     return Params(iks_scale, iso, pca_iso, suprep, dose_fmol, sca, clamp)
 
 
+class Track(C.Structure):
+    _fields_ = [("act", C.c_double), ("rep", C.c_double), ("vrest", C.c_double),
+                ("vmax", C.c_double), ("vmin", C.c_double), ("flags", C.c_int)]
+
+
 class Diag(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("Iion", "dV_dt", "ICaL", "IKs", "IKr", "IK1", "INa")]
 
@@ -81,6 +86,16 @@ def lib():
                                     C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                     C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int64,
                                     C.POINTER(C.c_int64), C.c_int, C.c_void_p, C.c_void_p]
+        L.or_strand_run2.restype = C.c_int64
+        L.or_strand_run2.argtypes = [C.c_int, u8, dp, C.POINTER(Params), C.c_double, C.c_double,
+                                     C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int64,
+                                     C.POINTER(C.c_int64), C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]
+        L.or_track_init.argtypes = [C.POINTER(Track)]
+        L.or_track_step.argtypes = [C.POINTER(Track), C.c_double, C.c_double]
+        L.or_track_result.restype = C.c_int
+        L.or_track_result.argtypes = [C.POINTER(Track), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_pseudo_ecg.argtypes = [C.c_int64, C.c_int, dp, C.c_double, C.c_double, dp]
         L.or_ghost_map.restype = C.c_int64
         L.or_ghost_map.argtypes = [C.c_int, C.c_int, C.c_int, u8, i32]
@@ -148,16 +163,50 @@ def strand_run(celltype, coeff, p=None, dx=0.15, dt=0.02, bcl=1000.0, stop_time=
     vrec = np.zeros((cap, n)) if record else None
     rt = np.zeros(cap) if record else None
     act = np.zeros(n)
+    apd = np.zeros(n)
+    ead = np.zeros(n, np.int32)
     fin = np.zeros((n, NSTATE))
     rows = C.c_int64(0)
     ab = int(act_beat) if act_beat is not None else int(stop_time // bcl) - 1
-    steps = lib().or_strand_run(n, celltype, coeff, C.byref(p), dx, dt, bcl, stop_time, stim,
-                                stim_dur, stim_start, nstim, _ptr(vrec), _ptr(rt),
-                                rec_every if record else 0, cap, C.byref(rows), ab, _ptr(act),
-                                _ptr(fin))
+    steps = lib().or_strand_run2(n, celltype, coeff, C.byref(p), dx, dt, bcl, stop_time, stim,
+                                 stim_dur, stim_start, nstim, _ptr(vrec), _ptr(rt),
+                                 rec_every if record else 0, cap, C.byref(rows), ab, _ptr(act),
+                                 _ptr(apd), _ptr(ead), _ptr(fin))
     r = rows.value
     return dict(v=None if vrec is None else vrec[:r], t=None if rt is None else rt[:r],
-                act=act, state=fin, steps=int(steps))
+                act=act, apd=apd, ead=ead, state=fin, steps=int(steps))
+
+
+def track_trace(t, v):
+    """Beat measurements of one V trace (oracle restatement of the strand
+    tracker): returns (act, apd90, ead_flags)."""
+    tr = Track()
+    lib().or_track_init(C.byref(tr))
+    for ti, vi in zip(np.asarray(t, float), np.asarray(v, float)):
+        lib().or_track_step(C.byref(tr), ti, vi)
+    a, d = C.c_double(), C.c_double()
+    e = lib().or_track_result(C.byref(tr), C.byref(a), C.byref(d))
+    return a.value, d.value, e
+
+
+def qt_from_phi(t, phi, t_q, t_qrs_end, t_end):
+    """QT of a pseudo-ECG (oracle restatement of the strand measurement):
+    T peak = largest |phi| in [t_qrs_end, t_end); T end = first sample from
+    the peak on with |phi| <= 5 % of it; QT = T end - t_q (-1 if none)."""
+    t = np.asarray(t, float)
+    a = np.abs(np.asarray(phi, float))
+    win = (t >= t_qrs_end) & (t < t_end)
+    if not win.any() or a[win].max() <= 0:
+        return -1.0
+    idx = np.flatnonzero(win)
+    ipk = idx[np.argmax(a[win])]
+    pk = a[ipk]
+    for j in range(ipk, t.size):
+        if t[j] >= t_end:
+            break
+        if a[j] <= 0.05 * pk:
+            return float(t[j] - t_q)
+    return -1.0
 
 
 def pseudo_ecg(v, dx=0.15, x0=35.0):

@@ -85,6 +85,34 @@ int64_t or_strand_run(int n, const uint8_t *celltype, const double *coeff,
                       int64_t *nrows, int act_beat, double *act,
                       double *final_state);
 
+/* Per-cell beat measurements inside beat k = floor(t/BCL) (the reference
+ * defines only the -30 mV activation, OneD.cc:422; APD90 and the EAD test
+ * are this build's definitions, restated here independently of the
+ * kernel):  rest = V at the first step of the beat; act = first t with
+ * V >= -30 mV; vmax = running peak after act; repolarised at the first t
+ * with V <= vmax - 0.9 (vmax - rest), APD90 = t_rep - act; EAD when, from
+ * act + 100 ms until repolarisation, V rises more than 2 mV above its
+ * running minimum; flag 2 when activated but not repolarised. */
+typedef struct {
+    double act, rep;
+    double vrest, vmax, vmin;
+    int flags; /* bit0 entered, bit1 activated, bit2 repolarised, bit3 EAD */
+} or_track;
+void or_track_init(or_track *t);
+void or_track_step(or_track *t, double time, double v);
+/* apd = t_rep - act or -1; returns the EAD flags (1 EAD, 2 no repolarisation). */
+int or_track_result(const or_track *t, double *act, double *apd);
+
+/* or_strand_run plus the beat measurements of every cell in beat
+ * `act_beat` (apd[n], ead[n]; either may be NULL). */
+int64_t or_strand_run2(int n, const uint8_t *celltype, const double *coeff,
+                       const or_params *p, double dx, double dt, double bcl,
+                       double stop_time, double stim_strength, double stim_dur,
+                       double stim_start, int nstim,
+                       double *vrec, double *rec_t, int rec_every, int64_t rec_cap,
+                       int64_t *nrows, int act_beat, double *act, double *apd, int *ead,
+                       double *final_state);
+
 /* Pseudo-ECG of PlotScript/plot1DECG_compare.py:89-105 over T rows of L cells. */
 void or_pseudo_ecg(int64_t T, int L, const double *v, double dx, double x0, double *phi);
 

@@ -569,18 +569,67 @@ int64_t or_single_cell(int celltype, const or_params *p, double bcl, int nbeats,
     return step;
 }
 
+void or_track_init(or_track *t)
+{
+    t->act = -1.0;
+    t->rep = -1.0;
+    t->vrest = 0.0;
+    t->vmax = -1e30;
+    t->vmin = 1e30;
+    t->flags = 0;
+}
+
+void or_track_step(or_track *t, double time, double v)
+{
+    if (!(t->flags & 1)) {
+        t->flags |= 1;
+        t->vrest = v;
+    }
+    if (!(t->flags & 2)) {
+        if (v >= -30.0) {
+            t->flags |= 2;
+            t->act = time;
+            t->vmax = v;
+        }
+        return;
+    }
+    if (t->flags & 4) return;
+    if (v > t->vmax) t->vmax = v;
+    double level = t->vmax - 0.9 * (t->vmax - t->vrest);
+    if (v <= level) {
+        t->flags |= 4;
+        t->rep = time;
+    } else if (time >= t->act + 100.0) {
+        if (v < t->vmin) t->vmin = v;
+        if (v > t->vmin + 2.0) t->flags |= 8;
+    }
+}
+
+int or_track_result(const or_track *t, double *act, double *apd)
+{
+    if (act) *act = (t->flags & 2) ? t->act : -1.0;
+    if (apd) *apd = (t->flags & 4) ? t->rep - t->act : -1.0;
+    int e = 0;
+    if (t->flags & 8) e |= 1;
+    if ((t->flags & 2) && !(t->flags & 4)) e |= 2;
+    return e;
+}
+
 /* OneD.cc:315-436 */
-int64_t or_strand_run(int n, const uint8_t *celltype, const double *coeff,
-                      const or_params *p, double dx, double dt, double bcl,
-                      double stopTime, double stimStrength, double stimDuration,
-                      double stimStart, int nstim_cells,
-                      double *vrec, double *rec_t, int rec_every, int64_t rec_cap,
-                      int64_t *nrows, int act_beat, double *act, double *final_state)
+int64_t or_strand_run2(int n, const uint8_t *celltype, const double *coeff,
+                       const or_params *p, double dx, double dt, double bcl,
+                       double stopTime, double stimStrength, double stimDuration,
+                       double stimStart, int nstim_cells,
+                       double *vrec, double *rec_t, int rec_every, int64_t rec_cap,
+                       int64_t *nrows, int act_beat, double *act, double *apd, int *ead,
+                       double *final_state)
 {
     double *s = (double *)malloc(sizeof(double) * OR_NSTATE * (size_t)n);
     double *oldV = (double *)malloc(sizeof(double) * (size_t)n);
     double *dcoeff = (double *)malloc(sizeof(double) * (size_t)n);
+    or_track *tr = (or_track *)malloc(sizeof(or_track) * (size_t)n);
     for (int i = 0; i < n; ++i) or_cell_init(s + (size_t)i * OR_NSTATE, -87.5);
+    for (int i = 0; i < n; ++i) or_track_init(&tr[i]);
     /* OneD.cc:121-131 */
     for (int i = 0; i < n; ++i) {
         if (n == 1) dcoeff[i] = 0.0;
@@ -588,8 +637,6 @@ int64_t or_strand_run(int n, const uint8_t *celltype, const double *coeff,
         else if (i == n - 1) dcoeff[i] = (coeff[i] - coeff[i - 1]) / dx;
         else dcoeff[i] = (coeff[i + 1] - coeff[i - 1]) / (2.0 * dx);
     }
-    if (act)
-        for (int i = 0; i < n; ++i) act[i] = -1.0;
     int64_t rows = 0;
     double time;
     int64_t step;
@@ -622,17 +669,35 @@ int64_t or_strand_run(int n, const uint8_t *celltype, const double *coeff,
             if (rec_t) rec_t[rows] = time;
             rows++;
         }
-        if (act && floor(time / bcl) == act_beat) {
-            for (int i = 0; i < n; ++i)
-                if (act[i] < 0 && s[(size_t)i * OR_NSTATE + OR_V] >= -30) act[i] = time;
-        }
+        if (floor(time / bcl) == act_beat)
+            for (int i = 0; i < n; ++i) or_track_step(&tr[i], time, s[(size_t)i * OR_NSTATE + OR_V]);
+    }
+    for (int i = 0; i < n; ++i) {
+        double a, d;
+        int e = or_track_result(&tr[i], &a, &d);
+        if (act) act[i] = a;
+        if (apd) apd[i] = d;
+        if (ead) ead[i] = e;
     }
     if (nrows) *nrows = rows;
     if (final_state) memcpy(final_state, s, sizeof(double) * OR_NSTATE * (size_t)n);
     free(s);
     free(oldV);
     free(dcoeff);
+    free(tr);
     return step;
+}
+
+int64_t or_strand_run(int n, const uint8_t *celltype, const double *coeff,
+                      const or_params *p, double dx, double dt, double bcl,
+                      double stopTime, double stimStrength, double stimDuration,
+                      double stimStart, int nstim_cells,
+                      double *vrec, double *rec_t, int rec_every, int64_t rec_cap,
+                      int64_t *nrows, int act_beat, double *act, double *final_state)
+{
+    return or_strand_run2(n, celltype, coeff, p, dx, dt, bcl, stopTime, stimStrength, stimDuration,
+                          stimStart, nstim_cells, vrec, rec_t, rec_every, rec_cap, nrows, act_beat,
+                          act, 0, 0, final_state);
 }
 
 /* PlotScript/plot1DECG_compare.py:89-105 */

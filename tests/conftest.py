@@ -1,0 +1,30 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+    config.addinivalue_line("markers", "slow: takes more than ~20 s")
+
+
+@pytest.fixture(scope="session")
+def lqt():
+    import lqt1_amd
+
+    if lqt1_amd.device_count() < 1:
+        pytest.fail("no HIP device: GPU tests must run on the GPU box")
+    return lqt1_amd
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import oracle as O
+
+    O.lib()
+    return O

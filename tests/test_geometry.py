@@ -1,0 +1,80 @@
+"""Ghost mirror map (GPU3D.cu:1487-1604) and synthetic geometries.
+
+The library's map (host entry lqt_geometry_ghost_map, the one used by
+lqt_tissue_create) must equal the oracle's restatement cell for cell, and
+both must give the hand-derived mirrors of simple shapes."""
+import numpy as np
+import pytest
+
+import lqt1_amd as A
+import oracle as O
+from lqt1_amd import geometry as G
+
+
+def _lab(mask):
+    return np.where(mask, A.ENDO, A.NONE).astype(np.uint8)
+
+
+def test_block_mirrors_known_answers():
+    """A tissue box: face ghosts mirror the adjacent face voxel, edge ghosts
+    the voxel diagonally inward, corner ghosts the corner voxel."""
+    m = np.zeros((6, 6, 6), bool)
+    m[1:5, 1:5, 1:5] = True
+    g, nv, ng = A.ghost_map(_lab(m))
+    assert nv == 64
+    idx = lambda x, y, z: g[x, y, z]
+    assert idx(0, 2, 3) == -idx(1, 2, 3)   # face -x
+    assert idx(5, 2, 3) == -idx(4, 2, 3)   # face +x
+    assert idx(2, 3, 0) == -idx(2, 3, 1)   # face -z
+    assert idx(0, 0, 3) == -idx(1, 1, 3)   # edge
+    assert idx(5, 0, 5) == -idx(4, 1, 4)   # corner
+    assert idx(0, 0, 0) == -idx(1, 1, 1)
+    # every non-tissue cell touching the box is a ghost; the outer ring of
+    # the 6^3 grid touches it everywhere
+    assert ng == 6 ** 3 - 64
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_library_map_equals_oracle(seed):
+    rng = np.random.default_rng(seed)
+    shape = (17, 13, 19)
+    x, y, z = np.meshgrid(*[np.linspace(-1, 1, n) for n in shape], indexing="ij")
+    r = x ** 2 + (y / 0.8) ** 2 + (z / 1.1) ** 2
+    m = (r < 0.8) & (r > 0.25 + 0.1 * rng.random()) & (rng.random(shape) > 0.05)
+    g_lib, nv, ng = A.ghost_map(_lab(m))
+    g_or, nv2 = O.ghost_map(m.astype(np.uint8))
+    assert nv == nv2
+    assert np.array_equal(g_lib, g_or)
+    assert ng == int((g_or < 0).sum())
+
+
+def test_every_stencil_neighbour_of_tissue_is_tissue_or_ghost():
+    rng = np.random.default_rng(5)
+    m = rng.random((12, 12, 12)) > 0.4
+    m[0, :, :] = m[-1, :, :] = m[:, 0, :] = m[:, -1, :] = m[:, :, 0] = m[:, :, -1] = False
+    g, _, _ = A.ghost_map(_lab(m))
+    for p in np.argwhere(m):
+        for o in np.argwhere(np.ones((3, 3, 3))) - 1:
+            if np.abs(o).sum() == 3:
+                continue  # the 19-point stencil uses faces and edges only
+            q = tuple(p + o)
+            assert g[q] != 0
+
+
+def test_annulus_counts():
+    geo = G.annulus_2d()
+    nv = geo.nvox
+    expect = np.pi * (40 ** 2 - 30 ** 2) / 0.15 ** 2
+    assert abs(nv - expect) / expect < 0.01
+    lab = geo.label[geo.label != A.NONE]
+    frac = [(lab == c).mean() for c in (A.ENDO, A.MCELL, A.EPI)]
+    assert abs(frac[0] - 0.30 * (30 + 1.5) / 35) < 0.05  # depth split by radius, area-weighted
+    assert 280 <= geo.info["s2_ci"] <= 340
+
+
+def test_label_by_depth_split():
+    d = np.linspace(0, 1, 1001)
+    lab = G.label_by_depth(d)
+    assert (lab == A.ENDO).mean() == pytest.approx(0.30, abs=0.002)
+    assert (lab == A.MCELL).mean() == pytest.approx(0.35, abs=0.002)
+    assert (lab == A.EPI).mean() == pytest.approx(0.35, abs=0.002)

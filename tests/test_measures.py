@@ -1,0 +1,81 @@
+"""Beat measurements the reference does not define (QT, APD90 per cell, EAD),
+pinned on synthetic waveforms with known answers, for both the library's
+QT (lqt_ecg_qt) and the oracle's restatements."""
+import numpy as np
+import pytest
+
+import lqt1_amd as A
+from lqt1_amd import _lib as L
+import oracle as O
+
+
+def _ap(t, t_act, apd90, rest=-87.0, peak=40.0, ead_at=None, ead_mv=0.0):
+    """Piecewise-linear AP: upstroke over 1 ms, plateau sloping to the 90 %
+    level at t_act + apd90, then back to rest; optional bump for an EAD."""
+    v = np.full_like(t, rest)
+    lvl = peak - 0.9 * (peak - rest)
+    up = (t >= t_act) & (t < t_act + 1.0)
+    v[up] = rest + (peak - rest) * (t[up] - t_act)
+    pl = (t >= t_act + 1.0) & (t <= t_act + apd90)
+    v[pl] = peak + (lvl - peak) * (t[pl] - t_act - 1.0) / (apd90 - 1.0)
+    dn = (t > t_act + apd90) & (t < t_act + apd90 + 20.0)
+    v[dn] = lvl + (rest - lvl) * (t[dn] - t_act - apd90) / 20.0
+    if ead_at is not None:
+        bump = np.exp(-0.5 * ((t - ead_at) / 5.0) ** 2) * ead_mv
+        v = v + np.where(pl, bump, 0.0)
+    return v
+
+
+def test_tracker_apd_and_activation():
+    t = np.arange(0, 1000, 0.02)
+    v = _ap(t, 60.0, 300.0)
+    act, apd, ead = O.track_trace(t, v)
+    # -30 mV is crossed at 60 + 57/127 ms on the upstroke
+    assert abs(act - (60.0 + 57.0 / 127.0)) <= 0.02 + 1e-9
+    assert abs(apd - (300.0 - 57.0 / 127.0)) <= 0.04
+    assert ead == 0
+
+
+def test_tracker_ead_and_failure():
+    t = np.arange(0, 1000, 0.02)
+    v = _ap(t, 60.0, 400.0, ead_at=300.0, ead_mv=8.0)
+    assert O.track_trace(t, v)[2] & 1
+    # a small bump (< 2 mV) is not an EAD, nor is the early dome
+    assert O.track_trace(t, _ap(t, 60.0, 400.0, ead_at=300.0, ead_mv=1.0))[2] == 0
+    assert O.track_trace(t, _ap(t, 60.0, 400.0, ead_at=100.0, ead_mv=8.0))[2] == 0
+    # never repolarising within the beat
+    w = _ap(t, 60.0, 2000.0)
+    assert O.track_trace(t, w)[2] & 2
+
+
+def _ecg(t, t_q, qrs_ms, t_peak, t_width, amp=0.06):
+    phi = np.zeros_like(t)
+    q = (t >= t_q) & (t < t_q + qrs_ms)
+    phi[q] = 0.28 * np.sin(np.pi * (t[q] - t_q) / qrs_ms)
+    phi += amp * np.exp(-0.5 * ((t - t_peak) / t_width) ** 2)
+    return phi
+
+
+@pytest.mark.parametrize("t_peak,width", [(330.0, 25.0), (420.0, 40.0)])
+def test_qt_known_answer(t_peak, width):
+    """T end is where the Gaussian T wave falls to 5 % of its peak:
+    t_peak + width*sqrt(2 ln 20)."""
+    dts = 0.2
+    t = np.arange(0, 1000, dts)
+    t_q = 50.0
+    phi = _ecg(t, t_q, 60.0, t_peak, width)
+    expect = t_peak + width * np.sqrt(2 * np.log(20.0)) - t_q
+    q_lib = L.ecg_qt(phi, 0.0, dts, t_q, t_q + 70.0, 1000.0)
+    q_or = O.qt_from_phi(t, phi, t_q, t_q + 70.0, 1000.0)
+    assert abs(q_lib - expect) <= dts + 1e-9
+    assert abs(q_or - q_lib) < 1e-9
+    # inverted T wave: same QT
+    q_inv = L.ecg_qt(np.where(t > t_q + 70, -phi, phi), 0.0, dts, t_q, t_q + 70.0, 1000.0)
+    assert abs(q_inv - q_lib) < 1e-9
+
+
+def test_qt_missing_t_end():
+    t = np.arange(0, 500, 0.2)
+    phi = np.where(t > 200, 0.05, 0.0)  # never returns
+    assert L.ecg_qt(phi, 0.0, 0.2, 50.0, 120.0, 500.0) == -1.0
+    assert O.qt_from_phi(t, phi, 50.0, 120.0, 500.0) == -1.0
