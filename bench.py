@@ -1,0 +1,221 @@
+"""Benchmark of the TPORd monodomain hot path on one MI355X.
+
+Workload (the metric's 3D configuration): synthetic biventricular shell on a
+256x256x320 grid at 0.2 mm (about 6.4M tissue voxels), rotating fibres
+(D_fibre 0.154, D_cross 0.0385 mm^2/ms), transmural endo/M/epi labels plus a
+5 mm M-cell island, IKs x0.2 with ISO on, apical endocardial stimulus
+(-52 pA/pF, 3 ms) at BCL 1000 ms, dt 0.02 ms.
+
+A step is one fused tissue step (diffusion + TPORd update of every voxel,
+plus the ghost refresh) with all state resident in HBM.  The headline value
+is FP32 cell-steps/s; the plain FP64 path is timed on the same geometry and
+steps, and a strand check gives the APD90 and activation-time error of FP32
+against FP64.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-strands]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def tissue_run(A, geo, prec, warmup, steps):
+    T = A.Tissue(geo.label, geo.fibre, geo.dx, 0.154, 0.0385, A.preset("config3"), prec, 0.02)
+    T.set_stimulus(geo.stim_sites, geo.stim_offsets, bcl=1000.0, duration=3.0, strength=-52.0)
+    T.step(warmup)
+    A.lib().lqt_device_synchronize()
+    t0 = time.perf_counter()
+    T.step(steps)  # synchronises at the end
+    wall = time.perf_counter() - t0
+    us_main, us_ghost = T.kernel_us()
+    out = dict(nvox=T.nvox, nghost=T.nghost, wall=wall, us_main=us_main, us_ghost=us_ghost,
+               bytes=T.bytes_per_step(), v=T.v(), guard=T.v_guard().tolist())
+    T.close()
+    return out
+
+
+def cpu_baseline(geo, steps, threads):
+    """The FP64 oracle's tissue step (OpenMP over voxels) on the same
+    geometry for a bounded number of steps."""
+    import numpy as np
+
+    import oracle as O
+
+    O.set_num_threads(threads)
+    lab = geo.label
+    mask = (lab != 255).astype(np.uint8)
+    g, nvox = O.ghost_map(mask)
+    cells = np.flatnonzero(mask.ravel())
+    fib = np.ascontiguousarray(geo.fibre.reshape(-1, 3)[cells], np.float32)
+    dten, drift = O.tissue_tensor(g, nvox, fib, 0.154, 0.0385, geo.dx)
+    del fib
+    ct = lab.ravel()[cells].astype(np.uint8)
+    s = np.tile(O.cell_init(), (nvox, 1))
+    ist = np.zeros(nvox)
+    ist[g.ravel()[geo.stim_sites] - 1] = -52.0
+    p = O.params(iks_scale=0.2, iso=1, pca_iso=4.0)
+    O.tissue_step(g, nvox, dten, drift, geo.dx, 0.02, ct, p, s, ist)  # first touch
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        O.tissue_step(g, nvox, dten, drift, geo.dx, 0.02, ct, p, s, ist)
+    wall = time.perf_counter() - t0
+    return nvox * steps / wall, wall
+
+
+def strand_accuracy(A):
+    """FP32 vs FP64 on the transmural strand of OneD.cc (100 cells, ISO off,
+    2 beats): worst activation-time and APD90 difference over all cells."""
+    import numpy as np
+
+    spec = [A.StrandSpec(A.transmural(100), A.preset("config1"), bcl=1000.0, nbeats=2)]
+    res = {}
+    for prec in (A.FP64, A.FP32):
+        st = A.Strands(spec, prec, 0.02)
+        st.run()
+        res[prec] = st.cell_results(0) + (st.results()[0]["qt"],)
+        st.close()
+    a64, p64, _, q64 = res[A.FP64]
+    a32, p32, _, q32 = res[A.FP32]
+    return dict(d_act_ms=float(np.max(np.abs(a32 - a64))), d_apd90_ms=float(np.max(np.abs(p32 - p64))),
+                d_qt_ms=float(abs(q32 - q64)))
+
+
+def strands_throughput(A, nstr, nbeats, seed=0):
+    """Batched strand sweep: nstr strands, length U{50..400}, IKs U[0,1],
+    ISO {0,1}, dose U[0,1]*300 fmol, BCL {600,800,1000,1500}, M fraction
+    U[0.2,0.5], nbeats beats, one persistent launch."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(nstr):
+        L = int(rng.integers(50, 401))
+        p = A.preset("cpu_default", iks_scale=float(rng.random()), iso=int(rng.integers(0, 2)), suprep=1,
+                     dose_fmol=float(300.0 * rng.random()))
+        mf = float(rng.uniform(0.2, 0.5))
+        specs.append(A.StrandSpec(A.transmural(L, 0.5 * (1 - mf), mf), p,
+                                  bcl=float(rng.choice([600, 800, 1000, 1500])), nbeats=nbeats))
+    st = A.Strands(specs, A.FP32, 0.02)
+    st.run()  # first run captures the graph
+    t0 = time.perf_counter()
+    st.run()
+    wall = time.perf_counter() - t0
+    cs = sum(s.celltype.size * s.nbeats * s.bcl / 0.02 for s in specs)
+    r = st.results()
+    out = dict(strands=nstr, beats=nbeats, wall_s=wall, strands_per_s=nstr / wall, cell_steps_per_s=cs / wall,
+               block=st.block(), ead=int(sum(x["ead"] for x in r)))
+    st.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--fp64-steps", type=int, default=0, help="default: --steps")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--no-strands", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (one GPU per box, "
+              "no multi-GPU path in this build)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import lqt1_amd as A
+    from lqt1_amd import geometry as G
+
+    if A.device_count() < 1:
+        raise SystemExit("no HIP device")
+    dev = A.device_name()
+    log("device", dev)
+    t0 = time.perf_counter()
+    geo = G.ventricle_3d(seed=args.seed)
+    log(f"geometry {time.perf_counter() - t0:.1f} s, {geo.nvox} voxels, {geo.stim_sites.size} stimulus sites")
+
+    r32 = tissue_run(A, geo, A.FP32, args.warmup, args.steps)
+    log(f"FP32: {r32['wall'] / args.steps * 1e3:.3f} ms/step, kernel {r32['us_main']:.1f} us, ghost {r32['us_ghost']:.1f} us")
+    k64 = args.fp64_steps or args.steps
+    r64 = tissue_run(A, geo, A.FP64, args.warmup, k64)
+    log(f"FP64: {r64['wall'] / k64 * 1e3:.3f} ms/step, kernel {r64['us_main']:.1f} us")
+
+    nvox = r32["nvox"]
+    v32 = nvox * args.steps / r32["wall"]
+    v64 = nvox * k64 / r64["wall"]
+    acc = dict(v_linf_mv_fp32_vs_fp64=float(np.max(np.abs(r32["v"] - r64["v"]))) if k64 == args.steps else None)
+    try:
+        acc.update(strand_accuracy(A))
+    except Exception as e:  # reported, not hidden
+        acc["strand_error"] = str(e)
+    achieved = r32["bytes"] / (r32["us_main"] * 1e-6) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel="tissue_step_kernel<float, true>", kernel_us=round(r32["us_main"], 2),
+                bytes_per_launch=r32["bytes"])
+
+    strands = None
+    if not args.no_strands:
+        try:
+            strands = strands_throughput(A, 512, 2)
+            log(f"strands: {strands}")
+        except Exception as e:
+            strands = dict(error=str(e))
+
+    cpu = None
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        threads = max(1, min(threads, 16))
+        try:
+            cps, wall = cpu_baseline(geo, args.cpu_steps, threads)
+            cpu = dict(value=cps, unit="cell-steps/s", cores=threads, kind="port",
+                       sample=f"FP64 oracle tissue step (diffusion + TPORd update, OpenMP over voxels), "
+                              f"{args.cpu_steps} steps of the same {nvox}-voxel ventricle, {wall:.1f} s")
+        except Exception as e:
+            cpu = dict(error=str(e))
+
+    line = {
+        "metric": "3D tissue cell-steps/s FP32 vs FP64 @dAPD90<=1ms; strands/s batched+surrogate",
+        "value": v32,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (ion accumulators f64)",
+        "data": "synthetic",
+        "config": {"workload": "3D synthetic ventricle 256x256x320 @0.2 mm, FP32 state + FP64 slow "
+                               "concentrations, anisotropic D_f 0.154/D_c 0.0385, IKs x0.2 ISO on, dt 0.02 ms",
+                   "voxels": nvox, "ghost_cells": r32["nghost"], "device": dev},
+        "fp64": {"value": v64, "unit": "cell-steps/s", "steps": k64, "ms_per_step": r64["wall"] / k64 * 1e3,
+                 "kernel_us": round(r64["us_main"], 2)},
+        "fp32_over_fp64": v32 / v64,
+        "accuracy": acc,
+        "strands": strands,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

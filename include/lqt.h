@@ -90,8 +90,9 @@ lqt_status lqt_cells_read_state_file(lqt_cells *c, int i, const char *path);
  * (each with its own parameter set): S1 pacing, then the statistics of the
  * last beat.  stats[n*10] = rest, dV/dt max, overshoot, amplitude, APD20,
  * APD25, APD50, APD75, APD80, APD90 (SingleCell.cc:220-232).  If vtrace is
- * not NULL, V of cell `trace_cell` every `trace_every` steps is written to
- * vtrace[trace_cap]; *ntrace receives the count. */
+ * not NULL, (V, ICaL, IKs) of cell `trace_cell` every `trace_every` steps
+ * (the columns of SingleCell.cc:117-139) are written to vtrace[3*trace_cap];
+ * *ntrace receives the number of samples. */
 typedef struct {
     double bcl, stim_start, stim_duration, stim_strength;
     int nbeats;
@@ -193,6 +194,8 @@ typedef struct {
     double rec_from;      /* ms; ECG recorded for time >= rec_from          */
     double x0;            /* electrode position (mm from cell 0)            */
     int measure_beat;     /* beat k = floor(t/bcl) in which act/APD/EAD are taken */
+    int vrec_every;       /* V of every cell every vrec_every steps from rec_from
+                             (the output file of OneD.cc:402-415); 0 = off     */
 } lqt_strand_desc;
 
 typedef struct {
@@ -220,6 +223,9 @@ lqt_status lqt_strands_cell_results(lqt_strands *s, int i, double *act, double *
 /* Pseudo-ECG samples of strand i; *n receives the count (call with
  * phi == NULL to query). */
 lqt_status lqt_strands_ecg(lqt_strands *s, int i, double *phi, double *t0, int64_t *n);
+/* Recorded V rows of strand i (rows*ncell, row-major), time of row 0;
+ * *nrows receives the count (call with v == NULL to query). */
+lqt_status lqt_strands_vtrace(lqt_strands *s, int i, double *v, double *t0, int64_t *nrows);
 /* Final 40-state vectors of strand i (ncell*40). */
 lqt_status lqt_strands_final_state(lqt_strands *s, int i, double *out);
 /* Device time of the last lqt_strands_run (ms, HIP events). */

@@ -61,7 +61,7 @@ class StrandDesc(C.Structure):
         ("dx", C.c_double), ("bcl", C.c_double), ("stop_time", C.c_double),
         ("stim_start", C.c_double), ("stim_duration", C.c_double), ("stim_strength", C.c_double),
         ("nstim", C.c_int), ("rec_every", C.c_int), ("rec_from", C.c_double), ("x0", C.c_double),
-        ("measure_beat", C.c_int),
+        ("measure_beat", C.c_int), ("vrec_every", C.c_int),
     ]
 
 
@@ -136,6 +136,7 @@ def lib():
     _sig(L, "lqt_strands_cell_results", [_VP, C.c_int, _VP, _VP, _VP])
     _sig(L, "lqt_strands_ecg", [_VP, C.c_int, _VP, P(C.c_double), P(C.c_int64)])
     _sig(L, "lqt_strands_final_state", [_VP, C.c_int, _VP])
+    _sig(L, "lqt_strands_vtrace", [_VP, C.c_int, _VP, P(C.c_double), P(C.c_int64)])
     _sig(L, "lqt_strands_last_ms", [_VP, P(C.c_double)])
     _sig(L, "lqt_strands_block", [_VP, P(C.c_int)])
     _sig(L, "lqt_ecg_qt", [_VP, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,

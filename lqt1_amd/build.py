@@ -69,7 +69,24 @@ def build(force: bool = False, jobs: int = 8) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_drivers()
     return LIB
+
+
+DRIVERS = {"lqt_single_cell": "single_cell.cpp", "lqt_oned": "one_d.cpp", "lqt_tissue3d": "tissue3d.cpp"}
+
+
+def build_drivers() -> None:
+    """The C++ programs mirroring SingleCell.cc, OneD.cc and GPU3D.cu, linked
+    against liblqt.so (found next to them through the rpath)."""
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    for exe, src in DRIVERS.items():
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", os.path.join(bindir, exe),
+               os.path.join(HERE, "drivers", src), "-L" + HERE, "-llqt", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
 
 
 if __name__ == "__main__":

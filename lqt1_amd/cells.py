@@ -61,11 +61,11 @@ class Cells:
         nt = C.c_int64(0)
         if trace_cell >= 0 and trace_every > 0:
             cap = int(nbeats * bcl / self.dt / trace_every) + 4
-            trace = np.zeros(cap)
+            trace = np.zeros((cap, 3))
         L.check(L.lib().lqt_cells_pace(self.h, C.byref(p), L.ptr(stats), int(trace_cell), int(trace_every),
-                                       L.ptr(trace), 0 if trace is None else trace.size, C.byref(nt)))
+                                       L.ptr(trace), 0 if trace is None else trace.shape[0], C.byref(nt)))
         if trace is not None:
-            trace = trace[: nt.value]
+            trace = trace[: nt.value]  # columns V, ICaL, IKs
         return stats, trace
 
     def close(self):

@@ -376,7 +376,7 @@ lqt_status lqt_cells_pace(lqt_cells* c, const lqt_pacing* pace, double* stats, i
         if (c->trace_cap < trace_cap) {
             hipFree(c->trace);
             c->trace = nullptr;
-            LQT_HIP(hipMalloc((void**)&c->trace, sizeof(double) * trace_cap));
+            LQT_HIP(hipMalloc((void**)&c->trace, sizeof(double) * 3 * trace_cap));
             c->trace_cap = trace_cap;
         }
     }
@@ -429,7 +429,7 @@ lqt_status lqt_cells_pace(lqt_cells* c, const lqt_pacing* pace, double* stats, i
     }
     if (vtrace) {
         const int64_t n = std::min<int64_t>(trace_cap, (step + trace_every - 1) / trace_every);
-        LQT_HIP(hipMemcpy(vtrace, c->trace, sizeof(double) * n, hipMemcpyDeviceToHost));
+        LQT_HIP(hipMemcpy(vtrace, c->trace, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
         if (ntrace) *ntrace = n;
     }
     return LQT_OK;

@@ -36,6 +36,10 @@ struct lqt_strands {
     float* phi = nullptr;
     int64_t* nphi = nullptr;
     double* phit0 = nullptr;
+    double* vrec = nullptr;
+    int64_t* nvrec = nullptr;
+    double* vrect0 = nullptr;
+    int64_t vrec_total = 0;
     double* act = nullptr;
     double* apd = nullptr;
     int* ead = nullptr;
@@ -51,7 +55,8 @@ struct lqt_strands {
     double last_ms = 0;
     bool ran = false;
     // host copies of the results of the last run
-    std::vector<double> h_act, h_apd, h_phit0;
+    std::vector<double> h_act, h_apd, h_phit0, h_vrect0;
+    std::vector<int64_t> h_nvrec;
     std::vector<int> h_ead;
     std::vector<int64_t> h_nphi, h_steps;
     std::vector<float> h_phi;
@@ -74,6 +79,9 @@ static void fill_sargs(lqt_strands* s, StrandsArgs<R>& a) {
     a.phi = s->phi;
     a.nphi = s->nphi;
     a.phit0 = s->phit0;
+    a.vrec = s->vrec;
+    a.nvrec = s->nvrec;
+    a.vrect0 = s->vrect0;
     a.act = s->act;
     a.apd = s->apd;
     a.ead = s->ead;
@@ -160,7 +168,7 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
         LQT_HIP(hipEventCreate(&s->e1));
         // per strand device descriptors and sample capacities
         s->hdev.resize(nstrands);
-        int64_t off = 0;
+        int64_t off = 0, voff = 0;
         for (int i = 0; i < nstrands; ++i) {
             const lqt_strand_desc& d = desc[i];
             StrandDev& h = s->hdev[i];
@@ -184,8 +192,17 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
             h.phi_off = off;
             h.phi_cap = cap;
             off += cap;
+            h.vrec_every = d.vrec_every;
+            h.pad_ = 0;
+            int64_t vcap = 0;
+            if (d.vrec_every > 0 && d.stop_time >= d.rec_from)
+                vcap = (int64_t)((d.stop_time - std::max(0.0, d.rec_from)) / dt / d.vrec_every) + 3;
+            h.vrec_off = voff;
+            h.vrec_cap = vcap;
+            voff += vcap * d.ncell;
         }
         s->phi_total = off;
+        s->vrec_total = voff;
         // longest first: the cost of a strand is its step count times its
         // wavefronts
         std::vector<int> ord(nstrands);
@@ -220,6 +237,9 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
         LQT_HIP(hipMalloc((void**)&s->phi, sizeof(float) * std::max<int64_t>(1, s->phi_total)));
         LQT_HIP(hipMalloc((void**)&s->nphi, sizeof(int64_t) * nstrands));
         LQT_HIP(hipMalloc((void**)&s->phit0, sizeof(double) * nstrands));
+        LQT_HIP(hipMalloc((void**)&s->vrec, sizeof(double) * std::max<int64_t>(1, s->vrec_total)));
+        LQT_HIP(hipMalloc((void**)&s->nvrec, sizeof(int64_t) * nstrands));
+        LQT_HIP(hipMalloc((void**)&s->vrect0, sizeof(double) * nstrands));
         LQT_HIP(hipMalloc((void**)&s->steps, sizeof(int64_t) * nstrands));
         LQT_HIP(hipMalloc((void**)&s->act, sizeof(double) * s->pitch));
         LQT_HIP(hipMalloc((void**)&s->apd, sizeof(double) * s->pitch));
@@ -259,6 +279,9 @@ lqt_status lqt_strands_destroy(lqt_strands* s) {
     hipFree(s->phi);
     hipFree(s->nphi);
     hipFree(s->phit0);
+    hipFree(s->vrec);
+    hipFree(s->nvrec);
+    hipFree(s->vrect0);
     hipFree(s->act);
     hipFree(s->apd);
     hipFree(s->ead);
@@ -299,6 +322,8 @@ lqt_status lqt_strands_run(lqt_strands* s) {
     s->h_nphi.resize(s->n);
     s->h_steps.resize(s->n);
     s->h_phit0.resize(s->n);
+    s->h_vrect0.resize(s->n);
+    s->h_nvrec.resize(s->n);
     s->h_phi.resize(std::max<int64_t>(1, s->phi_total));
     LQT_HIP(hipMemcpy(s->h_act.data(), s->act, sizeof(double) * s->ncells, hipMemcpyDeviceToHost));
     LQT_HIP(hipMemcpy(s->h_apd.data(), s->apd, sizeof(double) * s->ncells, hipMemcpyDeviceToHost));
@@ -306,6 +331,8 @@ lqt_status lqt_strands_run(lqt_strands* s) {
     LQT_HIP(hipMemcpy(s->h_nphi.data(), s->nphi, sizeof(int64_t) * s->n, hipMemcpyDeviceToHost));
     LQT_HIP(hipMemcpy(s->h_steps.data(), s->steps, sizeof(int64_t) * s->n, hipMemcpyDeviceToHost));
     LQT_HIP(hipMemcpy(s->h_phit0.data(), s->phit0, sizeof(double) * s->n, hipMemcpyDeviceToHost));
+    LQT_HIP(hipMemcpy(s->h_vrect0.data(), s->vrect0, sizeof(double) * s->n, hipMemcpyDeviceToHost));
+    LQT_HIP(hipMemcpy(s->h_nvrec.data(), s->nvrec, sizeof(int64_t) * s->n, hipMemcpyDeviceToHost));
     if (s->phi_total > 0)
         LQT_HIP(hipMemcpy(s->h_phi.data(), s->phi, sizeof(float) * s->phi_total, hipMemcpyDeviceToHost));
     s->ran = true;
@@ -376,6 +403,18 @@ lqt_status lqt_strands_ecg(lqt_strands* s, int i, double* phi, double* t0, int64
     if (t0) *t0 = s->h_phit0[i];
     if (phi)
         for (int64_t j = 0; j < s->h_nphi[i]; ++j) phi[j] = s->h_phi[s->hdev[i].phi_off + j];
+    return LQT_OK;
+}
+
+lqt_status lqt_strands_vtrace(lqt_strands* s, int i, double* v, double* t0, int64_t* nrows) {
+    LQT_CHECK(s && i >= 0 && i < s->n && nrows, LQT_ERR_ARG, "lqt_strands_vtrace: bad argument");
+    LQT_CHECK(s->ran, LQT_ERR_STATE, "lqt_strands_vtrace: call lqt_strands_run first");
+    *nrows = s->h_nvrec[i];
+    if (t0) *t0 = s->h_vrect0[i];
+    if (v && s->h_nvrec[i] > 0) {
+        const int64_t cnt = s->h_nvrec[i] * s->desc[i].ncell;
+        LQT_HIP(hipMemcpy(v, s->vrec + s->hdev[i].vrec_off, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+    }
     return LQT_OK;
 }
 

@@ -56,7 +56,11 @@ __global__ __launch_bounds__(256) void cells_pace_kernel(int n, SoA<R> st, R* __
         const int64_t step = a.step0 + i;
         if (a.trace && k == a.trace_cell && a.trace_every > 0 && step % a.trace_every == 0) {
             const int64_t slot = step / a.trace_every;
-            if (slot < a.trace_cap) a.trace[slot] = V;
+            if (slot < a.trace_cap) {
+                a.trace[3 * slot] = V;
+                a.trace[3 * slot + 1] = dg.ICaL;
+                a.trace[3 * slot + 2] = dg.IKs;
+            }
         }
         const double absdvdt = fabs(dg.dV_dt);
         if (floor((time - a.stim_start) / a.bcl) == a.numS1 - 2 &&

@@ -73,8 +73,8 @@ __global__ __launch_bounds__(B) void strands_kernel(StrandsArgs<R> A) {
         CellTrack tr;
         tr.act = -1; tr.rep = -1; tr.vrest = 0; tr.vmax = -1e30; tr.vmin = 1e30; tr.flags = 0;
 
-        int64_t nrec = 0;
-        int pend = 0;          // a sample waits for its reduction
+        int64_t nrec = 0, nvrow = 0;
+        int pend = 0;         // a sample waits for its reduction
         int fin_slot = -1;     // slot whose per-wave partials wait for thread 0
         int64_t fin_idx = 0;
         int64_t pend_idx = 0;
@@ -160,6 +160,11 @@ __global__ __launch_bounds__(B) void strands_kernel(StrandsArgs<R> A) {
                 pend = 1;
                 pend_idx = nrec++;
             }
+            if (D.vrec_every > 0 && (step % D.vrec_every) == 0 && time >= D.rec_from && nvrow < D.vrec_cap) {
+                if (live) A.vrec[D.vrec_off + nvrow * (int64_t)n + tid] = (double)v;
+                if (nvrow == 0 && tid == 0) A.vrect0[sidx] = time;
+                ++nvrow;
+            }
             b ^= 1;
         }
 
@@ -174,6 +179,7 @@ __global__ __launch_bounds__(B) void strands_kernel(StrandsArgs<R> A) {
             A.finv[cell] = v;
         }
         if (tid == 0) {
+            A.nvrec[sidx] = nvrow;
             A.nphi[sidx] = nrec;
             A.steps[sidx] = step;
         }

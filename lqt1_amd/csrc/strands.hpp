@@ -7,9 +7,10 @@ namespace lqt {
 
 struct StrandDev {
     int ncell, cell_off, pidx, nstim;
-    int rec_every, measure_beat;
+    int rec_every, measure_beat, vrec_every, pad_;
     double dx, bcl, stop_time, stim_start, stim_dur, stim_strength, rec_from, x0;
     int64_t phi_off, phi_cap;
+    int64_t vrec_off, vrec_cap;  // rows
 };
 
 // Per-cell beat tracker: -30 mV activation (OneD.cc:422), running peak,
@@ -35,6 +36,9 @@ struct StrandsArgs {
     float* phi;                // pseudo-ECG samples
     int64_t* nphi;             // per strand sample count
     double* phit0;             // per strand time of the first sample
+    double* vrec;              // recorded V rows
+    int64_t* nvrec;            // per strand recorded rows
+    double* vrect0;            // per strand time of the first row
     double* act;               // per cell
     double* apd;               // per cell APD90
     int* ead;                  // per cell flags: 1 EAD, 2 no repolarisation

@@ -74,32 +74,33 @@ def annulus_2d(n=600, dx=0.15, r_out=40.0, wall=10.0, split=(0.30, 0.35, 0.35), 
                     s2.astype(np.int64), depth[:, :, None], dict(s2_ci=ci, r_in=r_in, r_out=r_out))
 
 
-def _shell_depth(P, a_in, c_in, a_out, c_out, iters=24):
+def _shell_depth(P, a_in, c_in, a_out, c_out, iters=16):
     """Depth s in [0,1] such that P lies on the ellipsoid with semi-axes
     (a(s), a(s), c(s)) interpolated between the inner and outer surfaces; -1
-    outside the shell.  Bisection on the monotone implicit function."""
+    outside the shell.  Bisection on the monotone implicit function, only
+    for the points inside the shell."""
     x, y, z = P
     rho2 = x * x + y * y
-
-    def f(s):
-        a = a_in + s * (a_out - a_in)
-        cc = c_in + s * (c_out - c_in)
-        return rho2 / (a * a) + z * z / (cc * cc) - 1.0
-
-    f0, f1 = f(0.0), f(1.0)
-    inside = (f0 >= 0) & (f1 <= 0)
-    lo = np.zeros_like(x)
-    hi = np.ones_like(x)
+    z2 = z * z
+    inside = (rho2 / (a_in * a_in) + z2 / (c_in * c_in) >= 1.0) & \
+             (rho2 / (a_out * a_out) + z2 / (c_out * c_out) <= 1.0)
+    r2 = rho2[inside]
+    zz = z2[inside]
+    lo = np.zeros(r2.shape, np.float32)
+    hi = np.ones(r2.shape, np.float32)
     for _ in range(iters):
         mid = 0.5 * (lo + hi)
-        fm = f(mid)
-        up = fm > 0
+        a = a_in + mid * (a_out - a_in)
+        cc = c_in + mid * (c_out - c_in)
+        up = r2 / (a * a) + zz / (cc * cc) > 1.0
         lo = np.where(up, mid, lo)
         hi = np.where(up, hi, mid)
-    return np.where(inside, 0.5 * (lo + hi), -1.0)
+    out = np.full(x.shape, -1.0, np.float32)
+    out[inside] = 0.5 * (lo + hi)
+    return out
 
 
-def ventricle_3d(shape=(256, 256, 320), dx=0.2, seed=0, lv=(12.0, 45.0, 22.0, 55.0),
+def ventricle_3d(shape=(256, 256, 320), dx=0.2, seed=0, lv=(10.5, 44.0, 20.5, 54.0),
                  rv_wall=4.0, base_mm=58.0, island_mm=5.0, split=(0.30, 0.35, 0.35),
                  fibre_epi=-60.0, fibre_endo=60.0, apex_stim_mm=4.0):
     """Biventricular shell: LV truncated ellipsoid (inner a, c; outer a, c in

@@ -27,6 +27,7 @@ class StrandSpec:
     stop_time: float | None = None  # default nbeats * bcl
     measure_beat: int | None = None  # default the last beat
     rec_all_beats: bool = False
+    vrec_every: int = 0             # V of every cell every vrec_every steps (0 = off)
     extra: dict = field(default_factory=dict)
 
 
@@ -55,7 +56,7 @@ class Strands:
             rec_from = 0.0 if s.rec_all_beats else mb * s.bcl
             x0 = L_ * s.dx + s.electrode_mm
             descs[i] = L.StrandDesc(L_, off, i, s.dx, s.bcl, stop, s.stim_start, s.stim_duration,
-                                    s.stim_strength, s.nstim, s.rec_every, rec_from, x0, mb)
+                                    s.stim_strength, s.nstim, s.rec_every, rec_from, x0, mb, s.vrec_every)
             cts.append(ct)
             coeffs.append(np.broadcast_to(np.asarray(s.coeff, np.float64), (L_,)))
             plist.append(s.params)
@@ -104,6 +105,15 @@ class Strands:
         d = self._descs[i]
         t = t0.value + np.arange(n.value) * d.rec_every * self.dt
         return t, phi
+
+    def vtrace(self, i):
+        n, t0 = C.c_int64(), C.c_double()
+        L.check(L.lib().lqt_strands_vtrace(self.h, i, None, C.byref(t0), C.byref(n)))
+        nc = self._descs[i].ncell
+        v = np.zeros((n.value, nc))
+        L.check(L.lib().lqt_strands_vtrace(self.h, i, L.ptr(v), C.byref(t0), C.byref(n)))
+        t = t0.value + np.arange(n.value) * self._descs[i].vrec_every * self.dt
+        return t, v
 
     def final_state(self, i):
         out = np.zeros((self._descs[i].ncell, 40))
