@@ -131,6 +131,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-strands", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--profile", choices=["fp32", "fp64", "copy"], default=None,
+                    help="short run of one kernel for rocprofv3 (no JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (one GPU per box, "
@@ -146,9 +148,20 @@ def main():
         raise SystemExit("no HIP device")
     dev = A.device_name()
     log("device", dev)
+    if args.profile == "copy":
+        # 2 GiB read + 2 GiB written per launch: the counters' reference
+        nbytes = 2 << 30
+        us = A._lib.bench_copy(nbytes, max(1, args.steps))
+        log(f"copy {nbytes} B: {us:.1f} us per copy, {2 * nbytes / us / 1e3:.0f} GB/s read+write")
+        return
     t0 = time.perf_counter()
     geo = G.ventricle_3d(seed=args.seed)
     log(f"geometry {time.perf_counter() - t0:.1f} s, {geo.nvox} voxels, {geo.stim_sites.size} stimulus sites")
+    if args.profile in ("fp32", "fp64"):
+        r = tissue_run(A, geo, A.FP32 if args.profile == "fp32" else A.FP64, args.warmup, args.steps)
+        log(f"{args.profile}: kernel {r['us_main']:.1f} us, ghost {r['us_ghost']:.1f} us, "
+            f"{r['bytes'] / (r['us_main'] * 1e-6) / 1e9:.0f} GB/s algorithmic, {r['bytes']:.0f} B per launch")
+        return
 
     r32 = tissue_run(A, geo, A.FP32, args.warmup, args.steps)
     log(f"FP32: {r32['wall'] / args.steps * 1e3:.3f} ms/step, kernel {r32['us_main']:.1f} us, ghost {r32['us_ghost']:.1f} us")

@@ -60,6 +60,10 @@ int lqt_device_count(void);
 lqt_status lqt_device_synchronize(void);
 /* Name of the GPU the library runs on (gcnArchName) into buf. */
 lqt_status lqt_device_name(char *buf, int len);
+/* No counterpart in the reference: times `reps` launches of a plain
+ * streaming copy of `bytes` bytes (device to device) and returns the mean
+ * time per copy; the reference point for HBM bandwidth and counters. */
+lqt_status lqt_bench_copy(int64_t bytes, int reps, double *us_per_copy);
 
 /* ---------------------------------------------------------------------------
  * Cells: n independent TPORd cells on the GPU, one lane per cell.

@@ -97,6 +97,7 @@ def lib():
     _sig(L, "lqt_device_count", [], C.c_int)
     _sig(L, "lqt_device_synchronize", [])
     _sig(L, "lqt_device_name", [C.c_char_p, C.c_int])
+    _sig(L, "lqt_bench_copy", [C.c_int64, C.c_int, P(C.c_double)])
     # cells
     _sig(L, "lqt_cells_create", [C.c_int, _VP, P(Params), C.c_int, C.c_int, C.c_double, P(_VP)])
     _sig(L, "lqt_cells_destroy", [_VP])
@@ -178,6 +179,13 @@ def device_name() -> str:
     buf = C.create_string_buffer(128)
     check(lib().lqt_device_name(buf, 128))
     return buf.value.decode()
+
+
+def bench_copy(nbytes, reps=20):
+    """Mean microseconds of one plain device copy of nbytes (lqt_bench_copy)."""
+    us = C.c_double()
+    check(lib().lqt_bench_copy(int(nbytes), int(reps), C.byref(us)))
+    return us.value
 
 
 def suprep_fractions(dose):

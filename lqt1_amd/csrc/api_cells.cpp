@@ -131,6 +131,38 @@ lqt_status lqt_device_synchronize(void) {
     return LQT_OK;
 }
 
+lqt_status lqt_bench_copy(int64_t bytes, int reps, double* us_per_copy) {
+    LQT_CHECK(bytes >= 16 && bytes % 16 == 0 && reps > 0 && us_per_copy, LQT_ERR_ARG,
+              "lqt_bench_copy: bytes must be a positive multiple of 16");
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    lqt_status rc = [&]() -> lqt_status {
+        LQT_HIP(hipMalloc(&a, bytes));
+        LQT_HIP(hipMalloc(&b, bytes));
+        LQT_HIP(hipStreamCreate(&st));
+        LQT_HIP(hipEventCreate(&e0));
+        LQT_HIP(hipEventCreate(&e1));
+        launch_fill(true, a, bytes / 4, 1.0, st);
+        launch_copy(b, a, bytes, st);  // first touch
+        LQT_HIP(hipEventRecord(e0, st));
+        for (int i = 0; i < reps; ++i) launch_copy(b, a, bytes, st);
+        LQT_HIP(hipEventRecord(e1, st));
+        LQT_HIP(hipGetLastError());
+        LQT_HIP(hipStreamSynchronize(st));
+        float ms = 0;
+        LQT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *us_per_copy = 1000.0 * ms / reps;
+        return LQT_OK;
+    }();
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    hipFree(a);
+    hipFree(b);
+    return rc;
+}
+
 lqt_status lqt_device_name(char* buf, int len) {
     LQT_CHECK(buf && len > 0, LQT_ERR_ARG, "lqt_device_name: bad buffer");
     int dev = 0;

@@ -72,5 +72,6 @@ void launch_tissue_step(bool fp32, bool aniso, const void* args, int64_t nvox, h
 void launch_ghost_fill(bool fp32, void* grid, const int32_t* dst, const int32_t* src, int64_t n,
                        hipStream_t st);
 void launch_fill(bool fp32, void* p, int64_t n, double value, hipStream_t st);
+void launch_copy(void* dst, const void* src, int64_t bytes, hipStream_t st);
 
 }  // namespace lqt

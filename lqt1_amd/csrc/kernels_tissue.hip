@@ -90,6 +90,20 @@ __global__ __launch_bounds__(256) void fill_kernel(R* __restrict__ p, int64_t n,
     if (i < n) p[i] = value;
 }
 
+// Plain streaming copy (16 B per lane): the reference point of the HBM
+// counters and of achievable bandwidth.
+__global__ __launch_bounds__(256) void copy_kernel(float4* __restrict__ dst, const float4* __restrict__ src,
+                                                   int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+void launch_copy(void* dst, const void* src, int64_t bytes, hipStream_t st) {
+    const int64_t n = bytes / 16;
+    const int B = 256;
+    copy_kernel<<<(n + B - 1) / B, B, 0, st>>>((float4*)dst, (const float4*)src, n);
+}
+
 void launch_tissue_step(bool fp32, bool aniso, const void* args, int64_t nvox, hipStream_t st) {
     const int B = 256;
     const int64_t G = (nvox + B - 1) / B;
