@@ -237,6 +237,32 @@ lqt_status lqt_strands_last_ms(const lqt_strands *s, double *ms);
 /* Work-group size chosen by the autotuner for this batch (64..512). */
 lqt_status lqt_strands_block(const lqt_strands *s, int *block);
 
+/* ---------------------------------------------------------------------------
+ * Surrogate: no counterpart in the reference (it has no learned component;
+ * its dose sweeps are separate 1D runs post-processed in Python,
+ * PlotScript/plot1DECG_multiple.py:55-99).  An MLP n_in -> hidden -> hidden
+ * -> n_reg + n_bce with GELU; BF16 operands on the matrix cores with FP32
+ * accumulation; squared error on the first n_reg outputs (weighted by a
+ * mask), binary cross-entropy with logits on the last n_bce outputs; Adam
+ * (beta 0.9/0.999, eps 1e-8) on FP32 master weights.
+ */
+typedef struct lqt_mlp lqt_mlp;
+lqt_status lqt_mlp_create(int n_in, int hidden, int n_reg, int n_bce, uint64_t seed, lqt_mlp **out);
+lqt_status lqt_mlp_destroy(lqt_mlp *m);
+/* Unpadded FP32 masters: w1[hidden*n_in], b1[hidden], w2[hidden*hidden],
+ * b2[hidden], w3[n_out*hidden], b3[n_out] (row = output unit). */
+lqt_status lqt_mlp_get_params(lqt_mlp *m, float *w1, float *b1, float *w2, float *b2, float *w3, float *b3);
+lqt_status lqt_mlp_set_params(lqt_mlp *m, const float *w1, const float *b1, const float *w2, const float *b2,
+                              const float *w3, const float *b3);
+/* y[n*n_out] = network(x[n*n_in]) (raw outputs; the BCE outputs are logits). */
+lqt_status lqt_mlp_forward(lqt_mlp *m, int64_t n, const float *x, float *y);
+/* Loss and parameter gradients of one batch of n rows, no update. */
+lqt_status lqt_mlp_grads(lqt_mlp *m, int n, const float *x, const float *y, const float *mask, float *loss,
+                         float *gw1, float *gb1, float *gw2, float *gb2, float *gw3, float *gb3);
+/* epochs passes of shuffled batches with Adam; loss_out[epochs*(n/batch)]. */
+lqt_status lqt_mlp_train(lqt_mlp *m, int64_t n, const float *x, const float *y, const float *mask, int epochs,
+                         int batch, double lr, uint64_t seed, float *loss_out);
+
 /* Host only: QT interval of a pseudo-ECG trace as lqt_strands_results
  * computes it (the reference never defines QT): samples
  * phi[j] at t0 + j*dts; Q at t_q; T peak = largest |phi| in

@@ -140,6 +140,13 @@ def lib():
     _sig(L, "lqt_strands_vtrace", [_VP, C.c_int, _VP, P(C.c_double), P(C.c_int64)])
     _sig(L, "lqt_strands_last_ms", [_VP, P(C.c_double)])
     _sig(L, "lqt_strands_block", [_VP, P(C.c_int)])
+    _sig(L, "lqt_mlp_create", [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, P(_VP)])
+    _sig(L, "lqt_mlp_destroy", [_VP])
+    _sig(L, "lqt_mlp_get_params", [_VP] + [_VP] * 6)
+    _sig(L, "lqt_mlp_set_params", [_VP] + [_VP] * 6)
+    _sig(L, "lqt_mlp_forward", [_VP, C.c_int64, _VP, _VP])
+    _sig(L, "lqt_mlp_grads", [_VP, C.c_int, _VP, _VP, _VP, _VP] + [_VP] * 6)
+    _sig(L, "lqt_mlp_train", [_VP, C.c_int64, _VP, _VP, _VP, C.c_int, C.c_int, C.c_double, C.c_uint64, _VP])
     _sig(L, "lqt_ecg_qt", [_VP, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                            P(C.c_double)])
     _lib = L

@@ -25,9 +25,11 @@ SOURCES = [
     "kernels_cells.hip",
     "kernels_tissue.hip",
     "kernels_strands.hip",
+    "kernels_mlp.hip",
     "api_cells.cpp",
     "api_tissue.cpp",
     "api_strands.cpp",
+    "api_mlp.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

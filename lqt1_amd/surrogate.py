@@ -1,0 +1,200 @@
+"""Dose-sweep surrogate: an MLP trained on strands the solver generates, used
+to steer a dense IKs-scale x gene-therapy-dose sweep, with every point near
+a decision threshold re-run by the batched strand solver.
+
+The reference has no learned component: its dose-response work is 16
+separate 1D runs post-processed in Python (PlotScript/plot1DECG_multiple.py:
+55-99).  Inputs (6): IKs scale, ISO, dose fraction, BCL, strand length,
+M-cell fraction.  Outputs (5): QT, APD90 of ENDO / M / EPI (squared error,
+masked where the solver gives no value) and the EAD logit (cross-entropy).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from .strands import StrandSpec, Strands, transmural
+
+N_IN, N_REG, N_BCE = 6, 4, 1
+DOSE_MAX_FMOL = 300.0  # the fitted dose range of fit_dose_dependent.py:122
+
+
+class MLP:
+    """lqt_mlp_*: BF16-MFMA MLP with FP32 accumulation and Adam."""
+
+    def __init__(self, n_in=N_IN, hidden=128, n_reg=N_REG, n_bce=N_BCE, seed=0):
+        h = C.c_void_p()
+        L.check(L.lib().lqt_mlp_create(n_in, hidden, n_reg, n_bce, seed, C.byref(h)))
+        self.h, self.n_in, self.hidden, self.n_out = h, n_in, hidden, n_reg + n_bce
+
+    def params(self):
+        H, ni, no = self.hidden, self.n_in, self.n_out
+        p = [np.zeros((H, ni), np.float32), np.zeros(H, np.float32), np.zeros((H, H), np.float32),
+             np.zeros(H, np.float32), np.zeros((no, H), np.float32), np.zeros(no, np.float32)]
+        L.check(L.lib().lqt_mlp_get_params(self.h, *[L.ptr(a) for a in p]))
+        return p
+
+    def set_params(self, p):
+        p = [np.ascontiguousarray(a, np.float32) for a in p]
+        L.check(L.lib().lqt_mlp_set_params(self.h, *[L.ptr(a) for a in p]))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], self.n_out), np.float32)
+        L.check(L.lib().lqt_mlp_forward(self.h, x.shape[0], L.ptr(x), L.ptr(y)))
+        return y
+
+    def grads(self, x, y, mask=None):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        mk = None if mask is None else np.ascontiguousarray(mask, np.float32)
+        H, ni, no = self.hidden, self.n_in, self.n_out
+        g = [np.zeros((H, ni), np.float32), np.zeros(H, np.float32), np.zeros((H, H), np.float32),
+             np.zeros(H, np.float32), np.zeros((no, H), np.float32), np.zeros(no, np.float32)]
+        loss = C.c_float()
+        L.check(L.lib().lqt_mlp_grads(self.h, x.shape[0], L.ptr(x), L.ptr(y), L.ptr(mk), C.byref(loss),
+                                      *[L.ptr(a) for a in g]))
+        return loss.value, g
+
+    def train(self, x, y, mask=None, epochs=100, batch=512, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        mk = None if mask is None else np.ascontiguousarray(mask, np.float32)
+        nb = x.shape[0] // batch
+        losses = np.zeros(epochs * nb, np.float32)
+        L.check(L.lib().lqt_mlp_train(self.h, x.shape[0], L.ptr(x), L.ptr(y), L.ptr(mk), epochs, batch, lr, seed,
+                                      L.ptr(losses)))
+        return losses
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().lqt_mlp_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+
+# ---- features and targets ---------------------------------------------------
+
+def features(iks, iso, dose, bcl, length, mfrac):
+    """Normalised network inputs (each roughly in [-1, 1])."""
+    iks, iso, dose, bcl, length, mfrac = np.broadcast_arrays(iks, iso, dose, bcl, length, mfrac)
+    return np.stack([2 * iks - 1, 2 * iso - 1, 2 * dose - 1, (bcl - 1000.0) / 500.0, (length - 225.0) / 175.0,
+                     (mfrac - 0.35) / 0.15], 1).astype(np.float32)
+
+
+T_SCALE, T_SHIFT = 100.0, 350.0
+
+
+def targets(results):
+    """(QT, APD90 x3) normalised + EAD flag; mask 0 where a value is missing."""
+    y = np.zeros((len(results), N_REG + N_BCE), np.float32)
+    m = np.ones_like(y)
+    for i, r in enumerate(results):
+        vals = [r["qt"]] + list(r["apd90"])
+        for j, v in enumerate(vals):
+            if v is None or v < 0:
+                m[i, j] = 0.0
+            else:
+                y[i, j] = (v - T_SHIFT) / T_SCALE
+        y[i, N_REG] = 1.0 if r["ead"] else 0.0
+    return y, m
+
+
+def decode(out):
+    """Network outputs -> (QT ms, APD90 ms x3, EAD probability)."""
+    qt = out[:, 0] * T_SCALE + T_SHIFT
+    apd = out[:, 1:N_REG] * T_SCALE + T_SHIFT
+    p = 1.0 / (1.0 + np.exp(-out[:, N_REG].astype(np.float64)))
+    return qt, apd, p
+
+
+# ---- solver runs --------------------------------------------------------------
+
+@dataclass
+class Point:
+    iks: float
+    iso: int
+    dose: float      # fraction of DOSE_MAX_FMOL
+    bcl: float
+    length: int
+    mfrac: float
+
+
+def spec_of(p: Point, nbeats: int) -> StrandSpec:
+    par = L.preset("cpu_default", iks_scale=float(p.iks), iso=int(p.iso), suprep=1,
+                   dose_fmol=float(p.dose) * DOSE_MAX_FMOL)
+    endo = 0.5 * (1.0 - p.mfrac)
+    return StrandSpec(transmural(int(p.length), endo, float(p.mfrac)), par, bcl=float(p.bcl), nbeats=nbeats)
+
+
+def simulate(points, nbeats=10, precision=L.FP32, chunk=4096):
+    """Run every point as a strand through the batched solver; returns the
+    per-strand results and the device time."""
+    out, t = [], 0.0
+    for i in range(0, len(points), chunk):
+        st = Strands([spec_of(p, nbeats) for p in points[i:i + chunk]], precision, 0.02)
+        t0 = time.perf_counter()
+        st.run()
+        t += time.perf_counter() - t0
+        out.extend(st.results())
+        st.close()
+    return out, t
+
+
+def random_points(n, seed=0):
+    """The training distribution: IKs U[0,1], ISO {0,1}, dose U[0,1], BCL
+    {600,800,1000,1500}, length U{50..400}, M fraction U[0.2,0.5]."""
+    rng = np.random.default_rng(seed)
+    return [Point(float(rng.random()), int(rng.integers(0, 2)), float(rng.random()),
+                  float(rng.choice([600.0, 800.0, 1000.0, 1500.0])), int(rng.integers(50, 401)),
+                  float(rng.uniform(0.2, 0.5))) for _ in range(n)]
+
+
+def points_features(points):
+    a = np.array([[p.iks, p.iso, p.dose, p.bcl, p.length, p.mfrac] for p in points], np.float64)
+    return features(*a.T)
+
+
+# ---- the steered sweep ---------------------------------------------------------
+
+@dataclass
+class SweepResult:
+    qt: np.ndarray            # final QT per grid point (solver value where re-checked)
+    ead: np.ndarray           # final EAD decision
+    long_qt: np.ndarray       # QT > threshold
+    rechecked: np.ndarray     # bool mask of points re-run by the solver
+    qt_err: np.ndarray        # surrogate QT error on the re-checked points (ms)
+    ead_agree: float          # fraction of re-checked points where the surrogate's EAD call matched
+    t_surrogate: float
+    t_recheck: float
+
+
+def sweep(mlp: MLP, iks_grid, dose_grid, iso_values=(0, 1), bcl=1000.0, length=100, mfrac=0.35,
+          qt_threshold=450.0, band_ms=10.0, p_band=(0.2, 0.8), nbeats=10):
+    """Surrogate over the grid, then the solver on every point within band_ms
+    of the QT threshold or with an EAD probability inside p_band."""
+    I, D, S = np.meshgrid(np.asarray(iks_grid, float), np.asarray(dose_grid, float), np.asarray(iso_values, float),
+                          indexing="ij")
+    flat = np.stack([I.ravel(), S.ravel(), D.ravel()], 1)
+    t0 = time.perf_counter()
+    x = features(flat[:, 0], flat[:, 1], flat[:, 2], bcl, length, mfrac)
+    qt, _, p = decode(mlp.forward(x))
+    t_sur = time.perf_counter() - t0
+    near = (np.abs(qt - qt_threshold) < band_ms) | ((p > p_band[0]) & (p < p_band[1]))
+    pts = [Point(float(a), int(s), float(d), bcl, length, mfrac) for a, s, d in flat[near]]
+    res, t_chk = simulate(pts, nbeats) if pts else ([], 0.0)
+    qt_f = qt.copy()
+    ead_f = p >= 0.5
+    err = np.zeros(len(res))
+    agree = 0
+    for k, (i, r) in enumerate(zip(np.flatnonzero(near), res)):
+        err[k] = qt[i] - r["qt"] if r["qt"] >= 0 else np.nan
+        agree += int((p[i] >= 0.5) == bool(r["ead"]))
+        qt_f[i] = r["qt"]
+        ead_f[i] = bool(r["ead"])
+    return SweepResult(qt_f, ead_f, qt_f > qt_threshold, near, err, agree / max(1, len(res)), t_sur, t_chk)
