@@ -23,27 +23,39 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
     return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
 }
 
+// Waves per SIMD the register allocator targets (2 holds the whole FP32
+// anisotropic state without spilling).
+#ifndef LQT_TISSUE_WAVES
+#define LQT_TISSUE_WAVES 2
+#endif
+
 template <class R, bool ANISO>
-__global__ __launch_bounds__(256) void tissue_step_kernel(TissueArgs<R> A) {
-    const int64_t k = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LQT_TISSUE_WAVES))) void tissue_step_kernel(
+    TissueArgs<R> A) {
+    // 32-bit byte offsets from uniform bases let the loads share address
+    // registers (voxel and padded-grid counts are bounded on the host)
+    const uint32_t k = (uint32_t)(xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
     if (k >= A.nvox) return;
-    const int64_t p = A.vpos[k];
+    constexpr uint32_t S = sizeof(R);
+    const uint32_t pb = (uint32_t)A.vpos[k] * S;  // byte offset of the voxel in the grid
     const R* __restrict__ vo = A.vold;
-    const int64_t sx = A.sx, sy = A.sy;
-    const R vc = vo[p];
-    const R vxp = vo[p + sx], vxm = vo[p - sx];
-    const R vyp = vo[p + sy], vym = vo[p - sy];
-    const R vzp = vo[p + 1], vzm = vo[p - 1];
+    const uint32_t sx = (uint32_t)A.sx * S, sy = (uint32_t)A.sy * S;
+    auto V = [&](uint32_t off) -> R { return *(const R*)((const char*)vo + off); };
+    const R vc = V(pb);
+    const R vxp = V(pb + sx), vxm = V(pb - sx);
+    const R vyp = V(pb + sy), vym = V(pb - sy);
+    const R vzp = V(pb + S), vzm = V(pb - S);
     R du;
     if (ANISO) {
         const R vxx = (vxm + vxp - R(2) * vc) * A.inv_h2;
         const R vyy = (vym + vyp - R(2) * vc) * A.inv_h2;
         const R vzz = (vzm + vzp - R(2) * vc) * A.inv_h2;
-        const R vxy = (vo[p + sx + sy] + vo[p - sx - sy] - vo[p - sx + sy] - vo[p + sx - sy]) * A.inv_4h2;
-        const R vxz = (vo[p + sx + 1] + vo[p - sx - 1] - vo[p - sx + 1] - vo[p + sx - 1]) * A.inv_4h2;
-        const R vyz = (vo[p + sy + 1] + vo[p - sy - 1] - vo[p - sy + 1] - vo[p + sy - 1]) * A.inv_4h2;
-        const R f0 = A.fib[k], f1 = A.fib[A.pitch + k], f2 = A.fib[2 * A.pitch + k];
-        const R b0 = A.drift[k], b1 = A.drift[A.pitch + k], b2 = A.drift[2 * A.pitch + k];
+        const R vxy = (V(pb + sx + sy) + V(pb - sx - sy) - V(pb - sx + sy) - V(pb + sx - sy)) * A.inv_4h2;
+        const R vxz = (V(pb + sx + S) + V(pb - sx - S) - V(pb - sx + S) - V(pb + sx - S)) * A.inv_4h2;
+        const R vyz = (V(pb + sy + S) + V(pb - sy - S) - V(pb - sy + S) - V(pb + sy - S)) * A.inv_4h2;
+        const uint32_t ok = k * S;
+        const R f0 = ld_nt(A.fib, ok), f1 = ld_nt(A.fib + A.pitch, ok), f2 = ld_nt(A.fib + 2 * A.pitch, ok);
+        const R b0 = ld_nt(A.drift, ok), b1 = ld_nt(A.drift + A.pitch, ok), b2 = ld_nt(A.drift + 2 * A.pitch, ok);
         const R dd = A.ddiff, dc = A.dcross;
         // D = Dc*I + (Df-Dc) f f^T ; du = sum_ij D_ij d_i d_j V + sum_j b_j d_j V
         du = (dc + dd * f0 * f0) * vxx + (dc + dd * f1 * f1) * vyy + (dc + dd * f2 * f2) * vzz +
@@ -65,7 +77,7 @@ __global__ __launch_bounds__(256) void tissue_step_kernel(TissueArgs<R> A) {
     const int c = A.cls[k];
     const R vn = tpord_update<R, false>(s, vc, c, *A.consts, A.dt, ist, du, nullptr);
     store_state(s, A.st, k);
-    A.vnew[p] = vn;
+    *(R*)((char*)A.vnew + pb) = vn;
 
     // V guard of GPU3D.cu:1796-1829, one atomic per wavefront.
     const bool lo = vn < R(-90), hi = vn > R(200), bad = !(vn == vn) || !(vn - vn == R(0));

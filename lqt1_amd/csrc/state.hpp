@@ -38,22 +38,49 @@ struct SoA {
     int64_t pitch;
 };
 
+// The state is streamed: each value is read once and written once per step
+// and the whole state of a large tissue is far larger than the caches, so
+// the accesses carry the nontemporal hint.  Addresses are a uniform slot
+// base plus a 32-bit byte offset, which the compiler turns into one shared
+// offset register for all 37 slots (the voxel count stays below 2^29).
+#ifndef LQT_STATE_NT
+#define LQT_STATE_NT 1
+#endif
+template <class T>
+__device__ __forceinline__ T ld_nt(const T* base, uint32_t byte_off) {
+#if LQT_STATE_NT
+    return __builtin_nontemporal_load((const T*)((const char*)base + byte_off));
+#else
+    return *(const T*)((const char*)base + byte_off);
+#endif
+}
+template <class T>
+__device__ __forceinline__ void st_nt(T v, T* base, uint32_t byte_off) {
+#if LQT_STATE_NT
+    __builtin_nontemporal_store(v, (T*)((char*)base + byte_off));
+#else
+    *(T*)((char*)base + byte_off) = v;
+#endif
+}
+
 template <class R>
-__device__ __forceinline__ void load_state(CellState<R>& s, const SoA<R>& a, int64_t k) {
-#define LQT_LD(n) s.n = a.r[(int64_t)RS_##n * a.pitch + k];
+__device__ __forceinline__ void load_state(CellState<R>& s, const SoA<R>& a, uint32_t k) {
+    const uint32_t orr = k * (uint32_t)sizeof(R), od = k * 8u;
+#define LQT_LD(n) s.n = ld_nt(a.r + (size_t)RS_##n * a.pitch, orr);
     LQT_RFIELDS(LQT_LD)
 #undef LQT_LD
-#define LQT_LD(n) s.n = a.d[(int64_t)DS_##n * a.pitch + k];
+#define LQT_LD(n) s.n = ld_nt(a.d + (size_t)DS_##n * a.pitch, od);
     LQT_DFIELDS(LQT_LD)
 #undef LQT_LD
 }
 
 template <class R>
-__device__ __forceinline__ void store_state(const CellState<R>& s, const SoA<R>& a, int64_t k) {
-#define LQT_ST(n) a.r[(int64_t)RS_##n * a.pitch + k] = s.n;
+__device__ __forceinline__ void store_state(const CellState<R>& s, const SoA<R>& a, uint32_t k) {
+    const uint32_t orr = k * (uint32_t)sizeof(R), od = k * 8u;
+#define LQT_ST(n) st_nt(s.n, a.r + (size_t)RS_##n * a.pitch, orr);
     LQT_RFIELDS(LQT_ST)
 #undef LQT_ST
-#define LQT_ST(n) a.d[(int64_t)DS_##n * a.pitch + k] = s.n;
+#define LQT_ST(n) st_nt(s.n, a.d + (size_t)DS_##n * a.pitch, od);
     LQT_DFIELDS(LQT_ST)
 #undef LQT_ST
 }
