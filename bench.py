@@ -109,14 +109,15 @@ def strands_throughput(A, nstr, nbeats, seed=0):
         specs.append(A.StrandSpec(A.transmural(L, 0.5 * (1 - mf), mf), p,
                                   bcl=float(rng.choice([600, 800, 1000, 1500])), nbeats=nbeats))
     st = A.Strands(specs, A.FP32, 0.02)
-    st.run()  # first run captures the graph
+    mode, trial = st.autotune(2000)  # also captures both graphs
     t0 = time.perf_counter()
     st.run()
     wall = time.perf_counter() - t0
     cs = sum(s.celltype.size * s.nbeats * s.bcl / 0.02 for s in specs)
     r = st.results()
     out = dict(strands=nstr, beats=nbeats, wall_s=wall, strands_per_s=nstr / wall, cell_steps_per_s=cs / wall,
-               block=st.block(), ead=int(sum(x["ead"] for x in r)))
+               layout_mode=mode, trial_ms=trial, max_block=st.block(), ead=int(sum(x["ead"] for x in r)),
+               qt_median_ms=float(np.median([x["qt"] for x in r if x["qt"] > 0])))
     st.close()
     return out
 
@@ -130,6 +131,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-strands", action="store_true")
+    ap.add_argument("--strand-beats", type=int, default=10, help="beats per strand of the 512-strand sweep")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--profile", choices=["fp32", "fp64", "copy"], default=None,
                     help="short run of one kernel for rocprofv3 (no JSON line)")
@@ -186,7 +188,7 @@ def main():
     strands = None
     if not args.no_strands:
         try:
-            strands = strands_throughput(A, 512, 2)
+            strands = strands_throughput(A, 512, args.strand_beats)
             log(f"strands: {strands}")
         except Exception as e:
             strands = dict(error=str(e))

@@ -234,8 +234,15 @@ lqt_status lqt_strands_vtrace(lqt_strands *s, int i, double *v, double *t0, int6
 lqt_status lqt_strands_final_state(lqt_strands *s, int i, double *out);
 /* Device time of the last lqt_strands_run (ms, HIP events). */
 lqt_status lqt_strands_last_ms(const lqt_strands *s, double *ms);
-/* Work-group size chosen by the autotuner for this batch (64..512). */
+/* Largest work-group size of the active launch layout (64..512). */
 lqt_status lqt_strands_block(const lqt_strands *s, int *block);
+/* Launch layouts: mode 0 runs one persistent kernel per block-size class
+ * (strands of 1..8 wavefronts, each class at its own block size, all classes
+ * as parallel branches of one graph); mode 1 runs every strand at the
+ * largest block.  The autotuner times both on the batch cut to trial_steps
+ * steps and keeps the faster (ms[2] receives the trial times). */
+lqt_status lqt_strands_autotune(lqt_strands *s, int trial_steps, int *mode, double *ms);
+lqt_status lqt_strands_set_mode(lqt_strands *s, int mode);
 
 /* ---------------------------------------------------------------------------
  * Surrogate: no counterpart in the reference (it has no learned component;

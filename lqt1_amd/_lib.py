@@ -140,6 +140,8 @@ def lib():
     _sig(L, "lqt_strands_vtrace", [_VP, C.c_int, _VP, P(C.c_double), P(C.c_int64)])
     _sig(L, "lqt_strands_last_ms", [_VP, P(C.c_double)])
     _sig(L, "lqt_strands_block", [_VP, P(C.c_int)])
+    _sig(L, "lqt_strands_autotune", [_VP, C.c_int, P(C.c_int), _VP])
+    _sig(L, "lqt_strands_set_mode", [_VP, C.c_int])
     _sig(L, "lqt_mlp_create", [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, P(_VP)])
     _sig(L, "lqt_mlp_destroy", [_VP])
     _sig(L, "lqt_mlp_get_params", [_VP] + [_VP] * 6)

@@ -21,14 +21,27 @@ struct lqt_strands {
     int64_t ncells = 0, pitch = 0;
     bool fp32 = true;
     double dt = 0.02;
-    int block = 512, grid = 1;
+    int block = 512;  // largest block of the active layout
+    // launch layouts: mode 0 = one persistent kernel per block-size class
+    // (strands of 1..8 wavefronts), mode 1 = one kernel at the largest block
+    struct Layout {
+        int block, grid, first, count;
+    };
+    std::vector<Layout> layout[2];
+    int mode = 0;
+    bool tuned = false;
+    double tune_ms[2] = {0, 0};
+    hipStream_t side[8] = {};
+    hipEvent_t fork = nullptr, join[8] = {};
+    hipGraph_t graph_m[2] = {nullptr, nullptr};
+    hipGraphExec_t exec_m[2] = {nullptr, nullptr};
     std::vector<lqt_strand_desc> desc;
     std::vector<uint8_t> label;  // per cell enum CellType
     std::vector<StrandDev> hdev;
     // device
     StrandDev* ddesc = nullptr;
-    int* order = nullptr;
-    int* queue = nullptr;
+    int* order = nullptr;  // [2][n]: the strand order of each mode
+    int* queue = nullptr;  // one work counter per layout entry
     uint8_t* cls = nullptr;
     double* coeff = nullptr;
     void* consts = nullptr;
@@ -49,8 +62,6 @@ struct lqt_strands {
     int64_t* steps = nullptr;
     int64_t phi_total = 0;
     hipStream_t stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double last_ms = 0;
     bool ran = false;
@@ -90,18 +101,68 @@ static void fill_sargs(lqt_strands* s, StrandsArgs<R>& a) {
     a.steps = s->steps;
 }
 
-static lqt_status launch_run(lqt_strands* s, hipStream_t st) {
-    launch_zero_int(s->queue, 1, st);
+static lqt_status launch_one(lqt_strands* s, int mode, int li, hipStream_t st) {
+    const lqt_strands::Layout& L = s->layout[mode][li];
     if (s->fp32) {
         StrandsArgs<float> a;
         fill_sargs(s, a);
-        launch_strands(true, s->block, s->grid, &a, st);
+        a.nstrands = L.count;
+        a.order = s->order + (size_t)mode * s->n + L.first;
+        a.queue = s->queue + li;
+        launch_strands(true, L.block, L.grid, &a, st);
     } else {
         StrandsArgs<double> a;
         fill_sargs(s, a);
-        launch_strands(false, s->block, s->grid, &a, st);
+        a.nstrands = L.count;
+        a.order = s->order + (size_t)mode * s->n + L.first;
+        a.queue = s->queue + li;
+        launch_strands(false, L.block, L.grid, &a, st);
     }
     LQT_HIP(hipGetLastError());
+    return LQT_OK;
+}
+
+// The whole run of a mode: zero the work counters (a kernel, no memset
+// node), then every layout entry as a parallel branch of the same graph.
+static lqt_status launch_run(lqt_strands* s, int mode, hipStream_t st) {
+    const int nl = (int)s->layout[mode].size();
+    launch_zero_int(s->queue, nl, st);
+    LQT_HIP(hipEventRecord(s->fork, st));
+    for (int li = 1; li < nl; ++li) {
+        LQT_HIP(hipStreamWaitEvent(s->side[li], s->fork, 0));
+        lqt_status e = launch_one(s, mode, li, s->side[li]);
+        if (e) return e;
+        LQT_HIP(hipEventRecord(s->join[li], s->side[li]));
+    }
+    lqt_status e = launch_one(s, mode, 0, st);
+    if (e) return e;
+    for (int li = 1; li < nl; ++li) LQT_HIP(hipStreamWaitEvent(st, s->join[li], 0));
+    return LQT_OK;
+}
+
+static lqt_status graph_for(lqt_strands* s, int mode) {
+    if (s->exec_m[mode]) return LQT_OK;
+    LQT_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    lqt_status e = launch_run(s, mode, s->stream);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(s->stream, &g);
+    if (e) return e;
+    LQT_HIP(ce);
+    s->graph_m[mode] = g;
+    LQT_HIP(hipGraphInstantiate(&s->exec_m[mode], g, nullptr, nullptr, 0));
+    return LQT_OK;
+}
+
+static lqt_status run_mode(lqt_strands* s, int mode, double* ms_out) {
+    lqt_status e = graph_for(s, mode);
+    if (e) return e;
+    LQT_HIP(hipEventRecord(s->e0, s->stream));
+    LQT_HIP(hipGraphLaunch(s->exec_m[mode], s->stream));
+    LQT_HIP(hipEventRecord(s->e1, s->stream));
+    LQT_HIP(hipStreamSynchronize(s->stream));
+    float ms = 0;
+    LQT_HIP(hipEventElapsedTime(&ms, s->e0, s->e1));
+    *ms_out = ms;
     return LQT_OK;
 }
 
@@ -161,9 +222,13 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
     s->dt = dt;
     s->desc.assign(desc, desc + nstrands);
     s->label.assign(celltype, celltype + ncells);
-    s->block = std::min(512, (maxl + 63) / 64 * 64);
     lqt_status st = [&]() -> lqt_status {
         LQT_HIP(hipStreamCreate(&s->stream));
+        for (int i = 0; i < 8; ++i) {
+            LQT_HIP(hipStreamCreate(&s->side[i]));
+            LQT_HIP(hipEventCreateWithFlags(&s->join[i], hipEventDisableTiming));
+        }
+        LQT_HIP(hipEventCreateWithFlags(&s->fork, hipEventDisableTiming));
         LQT_HIP(hipEventCreate(&s->e0));
         LQT_HIP(hipEventCreate(&s->e1));
         // per strand device descriptors and sample capacities
@@ -203,15 +268,37 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
         }
         s->phi_total = off;
         s->vrec_total = voff;
-        // longest first: the cost of a strand is its step count times its
-        // wavefronts
-        std::vector<int> ord(nstrands);
-        std::iota(ord.begin(), ord.end(), 0);
-        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
-            const double ca = desc[a].stop_time * ((desc[a].ncell + 63) / 64);
-            const double cb = desc[b].stop_time * ((desc[b].ncell + 63) / 64);
-            return ca > cb;
-        });
+        // Mode 1: every strand at the largest block, longest first (cost =
+        // step count x wavefronts).  Mode 0: strands grouped by their own
+        // wavefront count (one block-size class each), longest first inside
+        // a class.
+        auto waves = [&](int i) { return (desc[i].ncell + 63) / 64; };
+        auto cost = [&](int i) { return desc[i].stop_time * waves(i); };
+        std::vector<int> ord((size_t)2 * nstrands);
+        int* o0 = ord.data();
+        int* o1 = ord.data() + nstrands;
+        std::iota(o1, o1 + nstrands, 0);
+        std::stable_sort(o1, o1 + nstrands, [&](int a, int b) { return cost(a) > cost(b); });
+        std::copy(o1, o1 + nstrands, o0);
+        std::stable_sort(o0, o0 + nstrands, [&](int a, int b) { return waves(a) > waves(b); });
+        int dev = 0;
+        LQT_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        LQT_HIP(hipGetDeviceProperties(&prop, dev));
+        auto grid_of = [&](int block, int count) {
+            const int occ = std::max(1, strands_occupancy(s->fp32, block));
+            return std::max(1, std::min(count, occ * prop.multiProcessorCount));
+        };
+        for (int i = 0; i < nstrands;) {
+            int j = i;
+            while (j < nstrands && waves(o0[j]) == waves(o0[i])) ++j;
+            const int b = 64 * waves(o0[i]);
+            s->layout[0].push_back({b, grid_of(b, j - i), i, j - i});
+            i = j;
+        }
+        const int bmax = std::min(512, (maxl + 63) / 64 * 64);
+        s->layout[1].push_back({bmax, grid_of(bmax, nstrands), 0, nstrands});
+        s->block = bmax;
         std::vector<uint8_t> cl(ncells);
         for (int64_t i = 0; i < ncells; ++i) cl[i] = (uint8_t)cell_class(celltype[i]);
         std::vector<double> co(ncells, 0.154);
@@ -221,7 +308,7 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
         lqt_status e;
         if ((e = upload(&s->ddesc, s->hdev.data(), nstrands))) return e;
         if ((e = upload(&s->order, ord.data(), nstrands))) return e;
-        if ((e = upload(&s->queue, (const int*)nullptr, 1))) return e;
+        if ((e = upload(&s->queue, (const int*)nullptr, 8))) return e;
         if ((e = upload(&s->cls, cl.data(), ncells))) return e;
         if ((e = upload(&s->coeff, co.data(), ncells))) return e;
         if ((e = upload(&s->v0, v0.data(), nparams))) return e;
@@ -247,13 +334,6 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
         LQT_HIP(hipMalloc(&s->fr, rsz(s) * NR * s->pitch));
         LQT_HIP(hipMalloc((void**)&s->fd, sizeof(double) * ND * s->pitch));
         LQT_HIP(hipMalloc(&s->fv, rsz(s) * s->pitch));
-        // resident workgroups: one persistent workgroup per slot
-        int dev = 0;
-        LQT_HIP(hipGetDevice(&dev));
-        hipDeviceProp_t prop;
-        LQT_HIP(hipGetDeviceProperties(&prop, dev));
-        const int occ = std::max(1, strands_occupancy(s->fp32, s->block));
-        s->grid = std::max(1, std::min(nstrands, occ * prop.multiProcessorCount));
         return LQT_OK;
     }();
     if (st) {
@@ -267,8 +347,15 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
 lqt_status lqt_strands_destroy(lqt_strands* s) {
     if (!s) return LQT_OK;
     if (s->stream) hipStreamSynchronize(s->stream);
-    if (s->exec) hipGraphExecDestroy(s->exec);
-    if (s->graph) hipGraphDestroy(s->graph);
+    for (int m = 0; m < 2; ++m) {
+        if (s->exec_m[m]) hipGraphExecDestroy(s->exec_m[m]);
+        if (s->graph_m[m]) hipGraphDestroy(s->graph_m[m]);
+    }
+    for (int i = 0; i < 8; ++i) {
+        if (s->side[i]) hipStreamDestroy(s->side[i]);
+        if (s->join[i]) hipEventDestroy(s->join[i]);
+    }
+    if (s->fork) hipEventDestroy(s->fork);
     hipFree(s->ddesc);
     hipFree(s->order);
     hipFree(s->queue);
@@ -298,22 +385,9 @@ lqt_status lqt_strands_destroy(lqt_strands* s) {
 
 lqt_status lqt_strands_run(lqt_strands* s) {
     LQT_CHECK(s, LQT_ERR_ARG, "lqt_strands_run: null handle");
-    if (!s->exec) {
-        LQT_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        lqt_status e = launch_run(s, s->stream);
-        hipGraph_t g = nullptr;
-        hipError_t ce = hipStreamEndCapture(s->stream, &g);
-        if (e) return e;
-        LQT_HIP(ce);
-        s->graph = g;
-        LQT_HIP(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
-    }
-    LQT_HIP(hipEventRecord(s->e0, s->stream));
-    LQT_HIP(hipGraphLaunch(s->exec, s->stream));
-    LQT_HIP(hipEventRecord(s->e1, s->stream));
-    LQT_HIP(hipStreamSynchronize(s->stream));
-    float ms = 0;
-    LQT_HIP(hipEventElapsedTime(&ms, s->e0, s->e1));
+    double ms = 0;
+    lqt_status e = run_mode(s, s->mode, &ms);
+    if (e) return e;
     s->last_ms = ms;
     // results to the host
     s->h_act.resize(s->ncells);
@@ -460,7 +534,43 @@ lqt_status lqt_ecg_qt(const double* phi, int64_t n, double t0, double dts, doubl
 
 lqt_status lqt_strands_block(const lqt_strands* s, int* block) {
     LQT_CHECK(s && block, LQT_ERR_ARG, "lqt_strands_block: bad argument");
-    *block = s->block;
+    int b = 0;
+    for (const auto& L : s->layout[s->mode]) b = std::max(b, L.block);
+    *block = b;
+    return LQT_OK;
+}
+
+lqt_status lqt_strands_autotune(lqt_strands* s, int trial_steps, int* mode, double* ms) {
+    LQT_CHECK(s && trial_steps > 0, LQT_ERR_ARG, "lqt_strands_autotune: bad argument");
+    // the same batch cut to trial_steps steps, timed in both layouts
+    std::vector<StrandDev> cut = s->hdev;
+    for (auto& d : cut) d.stop_time = std::min(d.stop_time, (trial_steps - 0.5) * s->dt);
+    LQT_HIP(hipMemcpy(s->ddesc, cut.data(), sizeof(StrandDev) * s->n, hipMemcpyHostToDevice));
+    double t[2] = {0, 0};
+    lqt_status e = LQT_OK;
+    for (int m = 0; m < 2 && !e; ++m) {
+        double w = 0;
+        e = run_mode(s, m, &w);  // first replay: warm
+        if (!e) e = run_mode(s, m, &t[m]);
+    }
+    LQT_HIP(hipMemcpy(s->ddesc, s->hdev.data(), sizeof(StrandDev) * s->n, hipMemcpyHostToDevice));
+    if (e) return e;
+    s->mode = t[0] <= t[1] ? 0 : 1;
+    s->tune_ms[0] = t[0];
+    s->tune_ms[1] = t[1];
+    s->tuned = true;
+    s->ran = false;
+    if (mode) *mode = s->mode;
+    if (ms) {
+        ms[0] = t[0];
+        ms[1] = t[1];
+    }
+    return LQT_OK;
+}
+
+lqt_status lqt_strands_set_mode(lqt_strands* s, int mode) {
+    LQT_CHECK(s && (mode == 0 || mode == 1), LQT_ERR_ARG, "lqt_strands_set_mode: mode must be 0 or 1");
+    s->mode = mode;
     return LQT_OK;
 }
 

@@ -80,6 +80,17 @@ class Strands:
         L.check(L.lib().lqt_strands_last_ms(self.h, C.byref(v)))
         return v.value
 
+    def autotune(self, trial_steps=2000):
+        """Time both launch layouts on the batch cut to trial_steps steps and
+        keep the faster; returns (mode, [ms mode 0, ms mode 1])."""
+        mode = C.c_int()
+        ms = np.zeros(2)
+        L.check(L.lib().lqt_strands_autotune(self.h, int(trial_steps), C.byref(mode), L.ptr(ms)))
+        return mode.value, ms.tolist()
+
+    def set_mode(self, mode):
+        L.check(L.lib().lqt_strands_set_mode(self.h, int(mode)))
+
     def block(self):
         v = C.c_int()
         L.check(L.lib().lqt_strands_block(self.h, C.byref(v)))

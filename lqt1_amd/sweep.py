@@ -1,0 +1,114 @@
+"""Surrogate-steered dose sweep, end to end on one GPU.
+
+    python -m lqt1_amd.sweep [--train 20000] [--beats 10] [--grid 100]
+                             [--epochs 200] [--plain] [--json out.json]
+
+1. Training data: `--train` strands at seeded random parameters (IKs U[0,1],
+   ISO {0,1}, dose U[0,1] x 300 fmol, BCL {600,800,1000,1500}, length
+   U{50..400}, M fraction U[0.2,0.5]) through the batched strand solver.
+2. The 6->128->128->5 GELU MLP is trained on the GPU (BF16 MFMA, FP32
+   accumulation, Adam).
+3. Sweep: grid x grid IKs scale x dose at ISO off/on (BCL 1000 ms, 100 cells,
+   M fraction 0.35); every point within 10 ms of the QT threshold or with an
+   EAD probability in (0.2, 0.8) is re-run by the solver.
+4. With `--plain`, the whole grid is also run by the solver, and the
+   steered decisions are compared with it point by point.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import surrogate as S
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=20000)
+    ap.add_argument("--beats", type=int, default=10)
+    ap.add_argument("--grid", type=int, default=100)
+    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--qt-threshold", type=float, default=None, help="default: median training QT at BCL 1000")
+    ap.add_argument("--plain", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args(argv)
+
+    def log(*x):
+        print(*x, file=sys.stderr, flush=True)
+
+    out = dict(train_strands=a.train, beats=a.beats, grid=a.grid)
+    pts = S.random_points(a.train, seed=a.seed)
+    t0 = time.perf_counter()
+    res, t_dev = S.simulate(pts, nbeats=a.beats)
+    t_gen = time.perf_counter() - t0
+    cs = sum(p.length * a.beats * p.bcl / 0.02 for p in pts)
+    out.update(gen_s=t_gen, gen_strands_per_s=a.train / t_gen, gen_cell_steps_per_s=cs / t_gen)
+    log(f"training data: {a.train} strands x {a.beats} beats in {t_gen:.1f} s "
+        f"({a.train / t_gen:.1f} strands/s, {cs / t_gen / 1e9:.2f} G cell-steps/s)")
+    x = S.points_features(pts)
+    y, m = S.targets(res)
+    out["ead_fraction"] = float(y[:, S.N_REG].mean())
+    net = S.MLP(seed=a.seed)
+    t0 = time.perf_counter()
+    losses = net.train(x, y, m, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed)
+    t_train = time.perf_counter() - t0
+    nb = a.train // a.batch
+    out.update(train_s=t_train, loss_first_epoch=float(losses[:nb].mean()), loss_last_epoch=float(losses[-nb:].mean()))
+    log(f"training: {a.epochs} epochs in {t_train:.2f} s, loss {losses[:nb].mean():.4f} -> {losses[-nb:].mean():.4f}")
+    # held-out error on fresh strands
+    hold = S.random_points(512, seed=a.seed + 1)
+    hres, _ = S.simulate(hold, nbeats=a.beats)
+    hy, hm = S.targets(hres)
+    qt_p, apd_p, p_p = S.decode(net.forward(S.points_features(hold)))
+    ok = hm[:, 0] > 0
+    out["heldout_qt_mae_ms"] = float(np.mean(np.abs(qt_p[ok] - (hy[ok, 0] * S.T_SCALE + S.T_SHIFT))))
+    out["heldout_ead_acc"] = float(np.mean((p_p >= 0.5) == (hy[:, S.N_REG] > 0.5)))
+    log(f"held-out: QT MAE {out['heldout_qt_mae_ms']:.2f} ms, EAD accuracy {out['heldout_ead_acc']:.3f}")
+
+    thr = a.qt_threshold
+    if thr is None:
+        q = np.array([r["qt"] for r, p in zip(res, pts) if p.bcl == 1000.0 and r["qt"] > 0])
+        thr = float(np.median(q)) if q.size else 450.0
+    g = np.linspace(0.0, 1.0, a.grid)
+    sw = S.sweep(net, g, g, (0, 1), qt_threshold=thr, band_ms=10.0, nbeats=a.beats)
+    n_pts = sw.qt.size
+    out.update(qt_threshold=thr, grid_points=int(n_pts), rechecked=int(sw.rechecked.sum()),
+               t_surrogate_s=sw.t_surrogate, t_recheck_s=sw.t_recheck,
+               recheck_qt_mae_ms=float(np.nanmean(np.abs(sw.qt_err))) if sw.qt_err.size else 0.0,
+               recheck_ead_agree=sw.ead_agree)
+    log(f"sweep: {n_pts} points, surrogate {sw.t_surrogate * 1e3:.1f} ms, re-checked {sw.rechecked.sum()} "
+        f"in {sw.t_recheck:.2f} s")
+    if a.plain:
+        I, D, SS = np.meshgrid(g, g, [0.0, 1.0], indexing="ij")
+        grid = [S.Point(float(i), int(s), float(d), 1000.0, 100, 0.35) for i, d, s in zip(I.ravel(), D.ravel(), SS.ravel())]
+        t0 = time.perf_counter()
+        plain, t_plain_dev = S.simulate(grid, nbeats=a.beats)
+        t_plain = time.perf_counter() - t0
+        qt_p = np.array([r["qt"] for r in plain])
+        ead_p = np.array([bool(r["ead"]) for r in plain])
+        chk = sw.rechecked
+        out.update(t_plain_s=t_plain,
+                   speedup=t_plain / (sw.t_surrogate + sw.t_recheck),
+                   rechecked_identical=bool(np.array_equal(sw.long_qt[chk], qt_p[chk] > thr) and
+                                            np.array_equal(sw.ead[chk], ead_p[chk])),
+                   agree_long_qt=float(np.mean(sw.long_qt == (qt_p > thr))),
+                   agree_ead=float(np.mean(sw.ead == ead_p)))
+        log(f"plain sweep {t_plain:.1f} s -> speed-up {out['speedup']:.1f}x, decisions agree "
+            f"{out['agree_long_qt']:.4f} (QT) {out['agree_ead']:.4f} (EAD), re-checked identical "
+            f"{out['rechecked_identical']}")
+    print(json.dumps(out), flush=True)
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump(out, f, indent=1)
+    return out
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,137 @@
+"""The benchmark configurations as parity cases.
+
+* Config 1, transmural strand (OneD.cc with ISO off, 5 beats): FP64 and FP32
+  strands against the oracle: activation, APD90, QT, final V.
+* Config 2, annular 2D sheet (600x600 at 0.15 mm, isotropic D): FP64 tissue
+  against the oracle for 150 steps from the endocardial S1; FP32 against
+  FP64 activation times over the first 40 ms of the S1 wave.
+* The C++ drivers (OneD.cc / SingleCell.cc mirrors) write the reference's
+  output files with the oracle's numbers.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def config1_oracle(oracle):
+    O = oracle
+    ct = np.array([O.ENDO] * 25 + [O.MCELL] * 35 + [O.EPI] * 40, np.uint8)
+    r = O.strand_run(ct, 0.154, O.params(iso=0), stop_time=5000.0, rec_every=10, act_beat=4)
+    phi = O.pseudo_ecg(r["v"], 0.15, 35.0)
+    keep = r["t"] >= 4000.0
+    r["qt"] = O.qt_from_phi(r["t"][keep], phi[keep], 4050.0, r["act"].max() + 10.0, 5000.0)
+    return r
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_config1_strand(lqt, config1_oracle, prec):
+    r = config1_oracle
+    spec = lqt.StrandSpec(lqt.transmural(100), lqt.preset("config1"), bcl=1000.0, nbeats=5)
+    st = lqt.Strands([spec], lqt.FP64 if prec == "fp64" else lqt.FP32, 0.02)
+    st.run()
+    act, apd, ead = st.cell_results(0)
+    res = st.results()[0]
+    dact = np.max(np.abs(act - r["act"]))
+    dapd = np.max(np.abs(apd - r["apd"]))
+    dv = np.max(np.abs(st.final_state(0)[:, 0] - r["state"][:, 0]))
+    print(f"config 1 {prec}: |d act| {dact:.3g} ms, |d APD90| {dapd:.3g} ms, QT {res['qt']:.2f} vs "
+          f"{r['qt']:.2f} ms, |dV_final| {dv:.3g} mV, CV {res['cv']:.4f} m/s")
+    assert res["steps"] == r["steps"]
+    if prec == "fp64":
+        assert dact < 1e-6 and dapd < 1e-6 and dv < 1e-6
+        assert abs(res["qt"] - r["qt"]) < 1e-6
+    else:
+        assert dact <= 0.02 + 1e-9 and dapd <= 1.0 and abs(res["qt"] - r["qt"]) <= 1.0
+    assert not ead.any()
+
+
+def _annulus_oracle(O, geo, nsteps, dt=0.02):
+    lab = geo.label
+    mask = (lab != 255).astype(np.uint8)
+    g, nvox = O.ghost_map(mask)
+    fib = np.zeros((nvox, 3), np.float32)
+    dten, drift = O.tissue_tensor(g, nvox, fib, 0.154, 0.154, geo.dx)
+    ct = lab.ravel()[np.flatnonzero(mask.ravel())].astype(np.uint8)
+    s = np.tile(O.cell_init(), (nvox, 1))
+    k = g.ravel()[geo.stim_sites] - 1
+    t = 0.0
+    p = O.params(iso=0)
+    for _ in range(nsteps):
+        ist = np.zeros(nvox)
+        if t - np.floor(t / 1000.0) * 1000.0 < 1.0:
+            ist[k] = -52.0
+        O.tissue_step(g, nvox, dten, drift, geo.dx, dt, ct, p, s, ist)
+        t += dt
+    return s
+
+
+def test_config2_annulus(lqt, oracle):
+    from lqt1_amd import geometry as G
+
+    geo = G.annulus_2d()
+    nsteps = 150
+    s = _annulus_oracle(oracle, geo, nsteps)
+    T = lqt.Tissue(geo.label, None, geo.dx, 0.154, 0.154, lqt.preset("config1"), lqt.FP64, 0.02)
+    T.set_stimulus(geo.stim_sites, None, bcl=1000.0, duration=1.0, strength=-52.0)
+    T.step(nsteps)
+    err = np.max(np.abs(T.v() - s[:, 0]))
+    print(f"config 2 FP64 vs oracle after {nsteps} steps: max |dV| {err:.3g} mV over {T.nvox} voxels")
+    assert err < 1e-7
+    # FP32 against FP64 on the S1 wave: activation step of every voxel
+    acts = {}
+    for prec in (lqt.FP64, lqt.FP32):
+        T = lqt.Tissue(geo.label, None, geo.dx, 0.154, 0.154, lqt.preset("config1"), prec, 0.02)
+        T.set_stimulus(geo.stim_sites, None, bcl=1000.0, duration=1.0, strength=-52.0)
+        a = np.full(T.nvox, -1)
+        for step in range(2000):
+            T.step(1)
+            v = T.v() if step % 5 == 4 else None
+            if v is not None:
+                new = (a < 0) & (v >= -30.0)
+                a[new] = step
+        acts[prec] = a
+    a64, a32 = acts[lqt.FP64], acts[lqt.FP32]
+    assert np.array_equal(a64 >= 0, a32 >= 0)
+    assert (a64 >= 0).mean() > 0.5
+    # sampled every 5 steps: identical sample means within 5 steps (0.1 ms)
+    assert np.max(np.abs(a64 - a32)) <= 5
+
+
+def test_oned_driver_writes_reference_file(lqt, oracle, tmp_path):
+    exe = os.path.join(ROOT, "lqt1_amd", "bin", "lqt_oned")
+    out = tmp_path / "oned.dat"
+    r = subprocess.run([exe, "--numS1", "1", "--stop", "1200", "--out", str(out)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    data = np.loadtxt(out)
+    O = oracle
+    ct = np.array([O.ENDO] * 25 + [O.MCELL] * 35 + [O.EPI] * 40, np.uint8)
+    ref = O.strand_run(ct, 0.154, O.params(), stop_time=1200.0, rec_every=10, act_beat=1)
+    assert data.shape == (ref["v"].shape[0], 101)
+    assert np.max(np.abs(data[:, 0] - ref["t"])) < 1e-9
+    assert np.max(np.abs(data[:, 1:] - ref["v"])) < 1e-6
+    assert "CV = " in r.stdout
+
+
+def test_single_cell_driver_stats(lqt, oracle, tmp_path):
+    exe = os.path.join(ROOT, "lqt1_amd", "bin", "lqt_single_cell")
+    r = subprocess.run([exe, "--beats", "3", "--outdir", str(tmp_path)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    st, _ = oracle.single_cell(oracle.MCELL, oracle.params(), nbeats=3)
+    txt = (tmp_path / "VentriSingleCellStats.dat").read_text()
+    apd90 = float([ln for ln in txt.splitlines() if ln.startswith("APD90")][0].split("=")[1].split()[0])
+    assert abs(apd90 - st["apd90"]) < 1e-4
+    rows = np.loadtxt(tmp_path / "VentriSingleCellResults_LQT1_ISO_MCELL.dat")
+    # one row per step of `for (time = 0; time < 3000; time += dt)`
+    n = 0
+    t = 0.0
+    while t < 3000.0:
+        t += 0.02
+        n += 1
+    assert rows.shape == (n, 4)

@@ -86,6 +86,29 @@ def test_strands_fp32_within_gates(lqt, oracle_runs):
         assert abs(res[i]["qt"] - r["qt"]) <= 1.0, (i, res[i]["qt"], r["qt"])
 
 
+def test_launch_layouts_agree_and_autotune(lqt):
+    """Both launch layouts (per-size classes in parallel branches, or one
+    kernel at the largest block) give identical per-strand results."""
+    specs = _specs(lqt)
+    st = lqt.Strands(specs, lqt.FP32, 0.02)
+    outs = []
+    for mode in (0, 1):
+        st.set_mode(mode)
+        st.run()
+        outs.append([st.cell_results(i) for i in range(len(specs))] + [st.ecg(i)[1] for i in range(len(specs))])
+    for a, b in zip(outs[0], outs[1]):
+        if isinstance(a, tuple):
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y)
+        else:
+            assert np.array_equal(a, b)
+    mode, ms = st.autotune(500)
+    assert mode in (0, 1) and ms[mode] <= ms[1 - mode]
+    st.run()
+    act, _, _ = st.cell_results(0)
+    assert np.array_equal(act, outs[0][0][0])
+
+
 def test_graph_replays_are_identical(lqt):
     specs = _specs(lqt)[1:3]
     st = lqt.Strands(specs, lqt.FP32, 0.02)
