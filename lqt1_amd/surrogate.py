@@ -132,9 +132,10 @@ def spec_of(p: Point, nbeats: int) -> StrandSpec:
     return StrandSpec(transmural(int(p.length), endo, float(p.mfrac)), par, bcl=float(p.bcl), nbeats=nbeats)
 
 
-def simulate(points, nbeats=10, precision=L.FP32, chunk=4096):
+def simulate(points, nbeats=10, precision=L.FP32, chunk=4096, progress=None):
     """Run every point as a strand through the batched solver; returns the
-    per-strand results and the device time."""
+    per-strand results and the device time.  progress(done, total) is
+    called after each chunk."""
     out, t = [], 0.0
     for i in range(0, len(points), chunk):
         st = Strands([spec_of(p, nbeats) for p in points[i:i + chunk]], precision, 0.02)
@@ -143,6 +144,8 @@ def simulate(points, nbeats=10, precision=L.FP32, chunk=4096):
         t += time.perf_counter() - t0
         out.extend(st.results())
         st.close()
+        if progress:
+            progress(len(out), len(points))
     return out, t
 
 

@@ -43,10 +43,13 @@ def main(argv=None):
     def log(*x):
         print(*x, file=sys.stderr, flush=True)
 
+    def prog(done, total):
+        log(f"  solver: {done}/{total} strands")
+
     out = dict(train_strands=a.train, beats=a.beats, grid=a.grid)
     pts = S.random_points(a.train, seed=a.seed)
     t0 = time.perf_counter()
-    res, t_dev = S.simulate(pts, nbeats=a.beats)
+    res, t_dev = S.simulate(pts, nbeats=a.beats, progress=prog)
     t_gen = time.perf_counter() - t0
     cs = sum(p.length * a.beats * p.bcl / 0.02 for p in pts)
     out.update(gen_s=t_gen, gen_strands_per_s=a.train / t_gen, gen_cell_steps_per_s=cs / t_gen)
@@ -89,7 +92,7 @@ def main(argv=None):
         I, D, SS = np.meshgrid(g, g, [0.0, 1.0], indexing="ij")
         grid = [S.Point(float(i), int(s), float(d), 1000.0, 100, 0.35) for i, d, s in zip(I.ravel(), D.ravel(), SS.ravel())]
         t0 = time.perf_counter()
-        plain, t_plain_dev = S.simulate(grid, nbeats=a.beats)
+        plain, t_plain_dev = S.simulate(grid, nbeats=a.beats, progress=prog)
         t_plain = time.perf_counter() - t0
         qt_p = np.array([r["qt"] for r in plain])
         ead_p = np.array([bool(r["ead"]) for r in plain])
