@@ -200,6 +200,13 @@ typedef struct {
     int measure_beat;     /* beat k = floor(t/bcl) in which act/APD/EAD are taken */
     int vrec_every;       /* V of every cell every vrec_every steps from rec_from
                              (the output file of OneD.cc:402-415); 0 = off     */
+    /* S2 of OneD_VW.cc:291-301: looked at only while the S1 window is
+     * closed; cells s2_first .. s2_first+s2_count-1 get s2_strength while
+     * time - s2_base lies in [s2_start, s2_start + s2_duration).  The
+     * reference: base (numS1-1)*BCL, duration 3, strength -104, cells 10..12.
+     * s2_duration 0 = no S2. */
+    double s2_base, s2_start, s2_duration, s2_strength;
+    int s2_first, s2_count;
 } lqt_strand_desc;
 
 typedef struct {
@@ -209,6 +216,9 @@ typedef struct {
     double act_first, act_last;/* ms, earliest/latest -30 mV crossing      */
     double cv;                 /* m/s between cells at 10% and 90% of length */
     int64_t steps;
+    double vmax_first, vmax_last; /* peak V (mV) of cell 0 / cell n-1 over the
+                                     steps with time - s2_base >= s2_start, from
+                                     -100 (maxEndo/maxEpi of OneD_VW.cc:206-207,390-394) */
 } lqt_strand_result;
 
 typedef struct lqt_strands lqt_strands;
@@ -243,6 +253,48 @@ lqt_status lqt_strands_block(const lqt_strands *s, int *block);
  * steps and keeps the faster (ms[2] receives the trial times). */
 lqt_status lqt_strands_autotune(lqt_strands *s, int trial_steps, int *mode, double *ms);
 lqt_status lqt_strands_set_mode(lqt_strands *s, int mode);
+
+/* ---------------------------------------------------------------------------
+ * Vulnerable-window search of OneD_VW.cc:199-449 (its MODE1/2/3), each S2
+ * timing a strand of one batched launch.  `base` is the strand of one
+ * candidate (cells, coefficients, S1 and the S2 stimulus; its s2_start is
+ * replaced per candidate).  A candidate is vulnerable when
+ * (vmax_first + boundary) * (vmax_last + boundary) <= 0 (OneD_VW.cc:410).
+ *   mode 2: s2 = left, left + step, ... (double accumulation, :405-423);
+ *           the first vulnerable one is the result;
+ *   mode 3: the same going down (:427-443);
+ *   mode 1: bisection of [left, right] (:396-404) while right - left >= step
+ *           or right == left, S2 too early (both peaks below -boundary)
+ *           moving left up, too late (both above) moving right down.
+ * Modes 2/3 run `batch` candidates per launch; mode 1 runs the bisection
+ * tree floor(log2(batch+1)) levels deep per launch.  The reference loops
+ * forever when no candidate is vulnerable: here the search stops after
+ * max_candidates visited candidates with found = 0 ("No VM!").
+ */
+typedef struct {
+    int mode;                  /* 1, 2 or 3 */
+    double left, right;        /* ms (OneD_VW.cc:201-202: 260, 260)         */
+    double step;               /* ms (0.1)                                  */
+    double boundary;           /* mV (10, :204)                             */
+    int batch;                 /* strands per launch                        */
+    int64_t max_candidates;
+} lqt_vw_desc;
+
+typedef struct {
+    int found;                 /* "VM found!"                               */
+    double s2;                 /* S2 start of the last visited candidate    */
+    double vmax_first, vmax_last; /* its maxEndo / maxEpi                   */
+    int64_t visited;           /* candidates the reference's loop runs      */
+    int64_t simulated;         /* strands run on the device                 */
+    int launches;
+    double device_ms;
+} lqt_vw_result;
+
+/* visited_s2[cap] and visited_vmax[2*cap] (either may be NULL) receive the
+ * visited sequence, as the reference prints it (s2, maxEndo, maxEpi). */
+lqt_status lqt_vw_search(const lqt_strand_desc *base, const uint8_t *celltype, const double *coeff,
+                         const lqt_params *params, lqt_precision prec, double dt, const lqt_vw_desc *vw,
+                         lqt_vw_result *out, double *visited_s2, double *visited_vmax, int64_t cap);
 
 /* ---------------------------------------------------------------------------
  * Surrogate: no counterpart in the reference (it has no learned component;

@@ -62,13 +62,26 @@ class StrandDesc(C.Structure):
         ("stim_start", C.c_double), ("stim_duration", C.c_double), ("stim_strength", C.c_double),
         ("nstim", C.c_int), ("rec_every", C.c_int), ("rec_from", C.c_double), ("x0", C.c_double),
         ("measure_beat", C.c_int), ("vrec_every", C.c_int),
+        ("s2_base", C.c_double), ("s2_start", C.c_double), ("s2_duration", C.c_double),
+        ("s2_strength", C.c_double), ("s2_first", C.c_int), ("s2_count", C.c_int),
     ]
+
+
+class VWDesc(C.Structure):
+    _fields_ = [("mode", C.c_int), ("left", C.c_double), ("right", C.c_double), ("step", C.c_double),
+                ("boundary", C.c_double), ("batch", C.c_int), ("max_candidates", C.c_int64)]
+
+
+class VWResult(C.Structure):
+    _fields_ = [("found", C.c_int), ("s2", C.c_double), ("vmax_first", C.c_double), ("vmax_last", C.c_double),
+                ("visited", C.c_int64), ("simulated", C.c_int64), ("launches", C.c_int),
+                ("device_ms", C.c_double)]
 
 
 class StrandResult(C.Structure):
     _fields_ = [("qt", C.c_double), ("apd90", C.c_double * 3), ("ead", C.c_int),
                 ("act_first", C.c_double), ("act_last", C.c_double), ("cv", C.c_double),
-                ("steps", C.c_int64)]
+                ("steps", C.c_int64), ("vmax_first", C.c_double), ("vmax_last", C.c_double)]
 
 
 _lib = None
@@ -142,6 +155,8 @@ def lib():
     _sig(L, "lqt_strands_block", [_VP, P(C.c_int)])
     _sig(L, "lqt_strands_autotune", [_VP, C.c_int, P(C.c_int), _VP])
     _sig(L, "lqt_strands_set_mode", [_VP, C.c_int])
+    _sig(L, "lqt_vw_search", [P(StrandDesc), _VP, _VP, P(Params), C.c_int, C.c_double, P(VWDesc), P(VWResult),
+                              _VP, _VP, C.c_int64])
     _sig(L, "lqt_mlp_create", [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, P(_VP)])
     _sig(L, "lqt_mlp_destroy", [_VP])
     _sig(L, "lqt_mlp_get_params", [_VP] + [_VP] * 6)

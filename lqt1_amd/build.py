@@ -30,6 +30,7 @@ SOURCES = [
     "api_tissue.cpp",
     "api_strands.cpp",
     "api_mlp.cpp",
+    "api_vw.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -75,7 +76,8 @@ def build(force: bool = False, jobs: int = 8) -> str:
     return LIB
 
 
-DRIVERS = {"lqt_single_cell": "single_cell.cpp", "lqt_oned": "one_d.cpp", "lqt_tissue3d": "tissue3d.cpp"}
+DRIVERS = {"lqt_single_cell": "single_cell.cpp", "lqt_oned": "one_d.cpp", "lqt_oned_vw": "one_d_vw.cpp",
+           "lqt_tissue3d": "tissue3d.cpp"}
 
 
 def build_drivers() -> None:

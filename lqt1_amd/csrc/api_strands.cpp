@@ -60,6 +60,7 @@ struct lqt_strands {
     double* fd = nullptr;
     void* fv = nullptr;
     int64_t* steps = nullptr;
+    double* vmax = nullptr;
     int64_t phi_total = 0;
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -70,6 +71,7 @@ struct lqt_strands {
     std::vector<int64_t> h_nvrec;
     std::vector<int> h_ead;
     std::vector<int64_t> h_nphi, h_steps;
+    std::vector<double> h_vmax;
     std::vector<float> h_phi;
 };
 
@@ -99,6 +101,7 @@ static void fill_sargs(lqt_strands* s, StrandsArgs<R>& a) {
     a.fin = SoA<R>{(R*)s->fr, s->fd, s->pitch};
     a.finv = (R*)s->fv;
     a.steps = s->steps;
+    a.vmax = s->vmax;
 }
 
 static lqt_status launch_one(lqt_strands* s, int mode, int li, hipStream_t st) {
@@ -211,6 +214,8 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
                   "strand %d: bad cell_offset or param_index", i);
         LQT_CHECK(d.dx > 0 && d.bcl > 0 && d.stop_time >= 0 && d.rec_every >= 0 && d.nstim >= 0, LQT_ERR_ARG,
                   "strand %d: bad dx, bcl, stop_time, rec_every or nstim", i);
+        LQT_CHECK(d.s2_duration >= 0 && d.s2_first >= 0 && d.s2_count >= 0, LQT_ERR_ARG,
+                  "strand %d: bad S2 duration or cells", i);
         ncells = std::max<int64_t>(ncells, (int64_t)d.cell_offset + d.ncell);
         maxl = std::max(maxl, d.ncell);
     }
@@ -251,6 +256,12 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
             h.stim_strength = d.stim_strength;
             h.rec_from = d.rec_from;
             h.x0 = d.x0;
+            h.s2_base = d.s2_base;
+            h.s2_start = d.s2_start;
+            h.s2_dur = d.s2_duration;
+            h.s2_strength = d.s2_strength;
+            h.s2_first = d.s2_first;
+            h.s2_count = d.s2_count;
             int64_t cap = 0;
             if (d.rec_every > 0 && d.stop_time >= d.rec_from)
                 cap = (int64_t)((d.stop_time - std::max(0.0, d.rec_from)) / dt / d.rec_every) + 3;
@@ -328,6 +339,7 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
         LQT_HIP(hipMalloc((void**)&s->nvrec, sizeof(int64_t) * nstrands));
         LQT_HIP(hipMalloc((void**)&s->vrect0, sizeof(double) * nstrands));
         LQT_HIP(hipMalloc((void**)&s->steps, sizeof(int64_t) * nstrands));
+        LQT_HIP(hipMalloc((void**)&s->vmax, sizeof(double) * 2 * nstrands));
         LQT_HIP(hipMalloc((void**)&s->act, sizeof(double) * s->pitch));
         LQT_HIP(hipMalloc((void**)&s->apd, sizeof(double) * s->pitch));
         LQT_HIP(hipMalloc((void**)&s->ead, sizeof(int) * s->pitch));
@@ -376,6 +388,7 @@ lqt_status lqt_strands_destroy(lqt_strands* s) {
     hipFree(s->fd);
     hipFree(s->fv);
     hipFree(s->steps);
+    hipFree(s->vmax);
     if (s->e0) hipEventDestroy(s->e0);
     if (s->e1) hipEventDestroy(s->e1);
     if (s->stream) hipStreamDestroy(s->stream);
@@ -399,6 +412,8 @@ lqt_status lqt_strands_run(lqt_strands* s) {
     s->h_vrect0.resize(s->n);
     s->h_nvrec.resize(s->n);
     s->h_phi.resize(std::max<int64_t>(1, s->phi_total));
+    s->h_vmax.resize(2 * (size_t)s->n);
+    LQT_HIP(hipMemcpy(s->h_vmax.data(), s->vmax, sizeof(double) * 2 * s->n, hipMemcpyDeviceToHost));
     LQT_HIP(hipMemcpy(s->h_act.data(), s->act, sizeof(double) * s->ncells, hipMemcpyDeviceToHost));
     LQT_HIP(hipMemcpy(s->h_apd.data(), s->apd, sizeof(double) * s->ncells, hipMemcpyDeviceToHost));
     LQT_HIP(hipMemcpy(s->h_ead.data(), s->ead, sizeof(int) * s->ncells, hipMemcpyDeviceToHost));
@@ -452,6 +467,8 @@ lqt_status lqt_strands_results(lqt_strands* s, lqt_strand_result* out) {
         r.qt = qt_from_ecg(s->h_phi.data() + s->hdev[i].phi_off, s->h_nphi[i], s->h_phit0[i], d.rec_every * s->dt,
                            t_q, t_qrs_end, t_end);
         r.steps = s->h_steps[i];
+        r.vmax_first = s->h_vmax[2 * (size_t)i];
+        r.vmax_last = s->h_vmax[2 * (size_t)i + 1];
         out[i] = r;
     }
     return LQT_OK;

@@ -80,6 +80,49 @@ constexpr Term IL[18] = {{18, 3}, {19, 2}, {20, 3}, {21, 2}, {22, 1}, {23, 2}, {
                          {-1, 3}, {-2, 2}, {-3, 3}, {-4, 2}, {-5, 1}, {-6, 2}, {-7, 3}, {-8, 2}, {-9, 3}};
 }  // namespace
 
+// The 2D sheet (nz == 1) of new_11.cu:1535-1604: 8 neighbours, the x and
+// y components of the tissue direction summed in the reference's order, the
+// nearest tissue neighbour searched y outer, x inner.  Cells beyond the grid
+// edge count as non-tissue (the reference reads past the edge for x+1 == X
+// and y+1 == Y, which is harmless there only when the tissue keeps a margin).
+static int64_t ghost_map_2d(int nx, int ny, std::vector<int32_t>& g) {
+    static const int ox[9] = {0, -1, 0, 1, -1, 1, -1, 0, 1};
+    static const int oy[9] = {0, -1, -1, -1, 0, 0, 1, 1, 1};
+    const double r2 = sqrt(2.0);
+    auto tis = [&](int x, int y) -> int {
+        return (x >= 0 && x < nx && y >= 0 && y < ny) ? g[(int64_t)x * ny + y] > 0 : 0;
+    };
+    int64_t ng = 0;
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y) {
+            if (g[(int64_t)x * ny + y] != 0) continue;
+            int gg[9];
+            int nb = 0;
+            for (int i = 1; i <= 8; ++i) nb += (gg[i] = tis(x + ox[i], y + oy[i]));
+            if (!nb) continue;
+            const double il = -gg[1] / r2 + gg[3] / r2 - gg[4] + gg[5] - gg[6] / r2 + gg[8] / r2;
+            const double ir = -gg[1] / r2 - gg[2] - gg[3] / r2 + gg[6] / r2 + gg[7] + gg[8] / r2;
+            double imax = fabs(il);
+            if (fabs(ir) > imax) imax = fabs(ir);
+            double nxv = il / imax, nyv = ir / imax;
+            if (imax - 0 < 0.0001) nxv = nyv = 0;
+            double best = 1000;
+            int bx = 0, by = 0;
+            for (int oyc = -1; oyc <= 1; ++oyc)
+                for (int oxc = -1; oxc <= 1; ++oxc) {
+                    const double dis = sqrt(1.0 * (oxc - nxv) * (oxc - nxv) + (oyc - nyv) * (oyc - nyv));
+                    if (tis(x + oxc, y + oyc) && dis < best) {
+                        best = dis;
+                        bx = oxc;
+                        by = oyc;
+                    }
+                }
+            g[(int64_t)x * ny + y] = -g[(int64_t)(x + bx) * ny + (y + by)];
+            ++ng;
+        }
+    return ng;
+}
+
 // g[c] = k+1 (tissue voxel k), -(k+1) (ghost mirroring voxel k), 0 otherwise.
 static void ghost_map(int nx, int ny, int nz, const uint8_t* label, std::vector<int32_t>& g, int64_t* nvox,
                       int64_t* nghost) {
@@ -88,6 +131,12 @@ static void ghost_map(int nx, int ny, int nz, const uint8_t* label, std::vector<
     int64_t num = 0;
     for (int64_t c = 0; c < ncell; ++c)
         if (label[c] != LQT_NONE) g[c] = (int32_t)(++num);
+    if (nz == 1) {
+        const int64_t ng = ghost_map_2d(nx, ny, g);
+        if (nvox) *nvox = num;
+        if (nghost) *nghost = ng;
+        return;
+    }
     const double wt[4] = {0.0, 1.0, sqrt(2.0), sqrt(3.0)};
     auto tis = [&](int x, int y, int z) -> int {
         if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) return 0;
@@ -463,7 +512,7 @@ lqt_status lqt_tissue_step_async(lqt_tissue* t, int64_t nsteps, void* stream) {
             a.s2_on = t->ns2 > 0 && t->proto.s2_duration > 0 && t->time >= t->proto.s2_time &&
                       t->time < t->proto.s2_time + t->proto.s2_duration;
             if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i], s));
-            launch_tissue_step(true, t->aniso, &a, t->nvox, s);
+            launch_tissue_step(true, t->aniso, t->nz == 1, &a, t->nvox, s);
         } else {
             TissueArgs<double> a;
             fill_args(t, a);
@@ -473,7 +522,7 @@ lqt_status lqt_tissue_step_async(lqt_tissue* t, int64_t nsteps, void* stream) {
             a.s2_on = t->ns2 > 0 && t->proto.s2_duration > 0 && t->time >= t->proto.s2_time &&
                       t->time < t->proto.s2_time + t->proto.s2_duration;
             if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i], s));
-            launch_tissue_step(false, t->aniso, &a, t->nvox, s);
+            launch_tissue_step(false, t->aniso, t->nz == 1, &a, t->nvox, s);
         }
         if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i + 1], s));
         LQT_HIP(hipGetLastError());
@@ -601,7 +650,7 @@ lqt_status lqt_tissue_bytes_per_step(const lqt_tissue* t, double* bytes) {
     // per voxel: V read + write, NR R-slots and ND doubles read + written,
     // class byte and grid index, fibre + drift (anisotropic)
     double per = 2.0 * (rs + NR * rs + ND * 8.0) + 1.0 + 4.0;
-    if (t->aniso) per += 6.0 * rs;
+    if (t->aniso) per += (t->nz == 1 ? 4.0 : 6.0) * rs;  // fibre + drift (x, y only in 2D)
     *bytes = per * (double)t->nvox;
     return LQT_OK;
 }

@@ -3,7 +3,8 @@
 // next strand from a work counter when done, and advances its strand with
 // the loop of OneD.cc:315-436: Jacobi snapshot of V in LDS, no-flux ends
 // by copy, S1 on the first cells by the in-cycle time of the double time
-// accumulator, then the TPORd update.  The pseudo-ECG of
+// accumulator (or, while that window is closed, the S2 of
+// OneD_VW.cc:291-301), then the TPORd update.  The pseudo-ECG of
 // PlotScript/plot1DECG_compare.py:89-105 is reduced across the workgroup
 // every rec_every steps (wavefront shuffles, then one LDS slot per wave).
 #include "strands.hpp"
@@ -19,8 +20,12 @@ __device__ __forceinline__ R wave_sum(R x) {
     return x;
 }
 
+// FP32: two waves per SIMD (state, trackers and step constants fit in 256
+// registers).  FP64 keeps its registers (one wave per SIMD for blocks up to
+// 256 lanes) rather than spilling to scratch.
 template <class R, int B>
-__global__ __launch_bounds__(B) void strands_kernel(StrandsArgs<R> A) {
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 2 : 1))) void strands_kernel(
+    StrandsArgs<R> A) {
     __shared__ R vbuf[2][B + 2];
     __shared__ float part[2][B / 64];
     __shared__ int s_next;
@@ -72,6 +77,10 @@ __global__ __launch_bounds__(B) void strands_kernel(StrandsArgs<R> A) {
 
         CellTrack tr;
         tr.act = -1; tr.rep = -1; tr.vrest = 0; tr.vmax = -1e30; tr.vmin = 1e30; tr.flags = 0;
+        // peak V after the S2 start (OneD_VW.cc:206-207,390-394); only
+        // cells 0 and n-1 report it
+        double wmax = -100.0;
+        const bool s2_cell = tid >= D.s2_first && tid < D.s2_first + D.s2_count;
 
         int64_t nrec = 0, nvrow = 0;
         int pend = 0;         // a sample waits for its reduction
@@ -123,9 +132,15 @@ __global__ __launch_bounds__(B) void strands_kernel(StrandsArgs<R> A) {
                 const R second = (vr + vl - R(2) * vc) * inv_dx2;
                 const R dvgap = RdD * first + RD * second;
                 const double tc = time - floor(time / D.bcl) * D.bcl;
-                const R ist = (tc >= D.stim_start && tc < D.stim_start + D.stim_dur && tid < D.nstim)
-                                  ? (R)D.stim_strength : R(0);
+                const double t2 = time - D.s2_base;
+                R ist = R(0);
+                if (tc >= D.stim_start && tc < D.stim_start + D.stim_dur) {
+                    if (tid < D.nstim) ist = (R)D.stim_strength;
+                } else if (t2 >= D.s2_start && t2 < D.s2_start + D.s2_dur && s2_cell) {
+                    ist = (R)D.s2_strength;
+                }
                 v = tpord_update<R, false>(s, v, c, cc, A.dt, ist, dvgap, nullptr);
+                if (t2 >= D.s2_start && (double)v >= wmax) wmax = (double)v;
 
                 // beat tracker, in the measured beat only (floor(t/BCL) == k)
                 if (floor(time / D.bcl) == D.measure_beat) {
@@ -177,6 +192,8 @@ __global__ __launch_bounds__(B) void strands_kernel(StrandsArgs<R> A) {
             A.ead[cell] = e;
             store_state(s, A.fin, cell);
             A.finv[cell] = v;
+            if (tid == 0) A.vmax[2 * (int64_t)sidx] = wmax;
+            if (tid == n - 1) A.vmax[2 * (int64_t)sidx + 1] = wmax;
         }
         if (tid == 0) {
             A.nvrec[sidx] = nvrow;

@@ -1,5 +1,6 @@
 // Fused tissue step: diffusion term from V^n (the reference's calc_du,
-// GPU3D.cu:370-385 and new_11.cu:352-417 in 3D form), then the TPORd update
+// GPU3D.cu:370-385 and new_11.cu:352-417 in 3D form; a grid with nz == 1 is
+// new_11.cu's 2D sheet, without z terms), then the TPORd update
 // of the voxel with that term (Itot GPU3D.cu:425-1220 and new_u_v
 // :1222-1230), one voxel per lane over the compacted tissue list.
 //
@@ -29,7 +30,8 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
 #define LQT_TISSUE_WAVES 2
 #endif
 
-template <class R, bool ANISO>
+// FLAT: a 2D sheet (nz == 1, new_11.cu:352-417), no z terms.
+template <class R, bool ANISO, bool FLAT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LQT_TISSUE_WAVES))) void tissue_step_kernel(
     TissueArgs<R> A) {
     // 32-bit byte offsets from uniform bases let the loads share address
@@ -44,9 +46,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LQT_TISSUE_
     const R vc = V(pb);
     const R vxp = V(pb + sx), vxm = V(pb - sx);
     const R vyp = V(pb + sy), vym = V(pb - sy);
-    const R vzp = V(pb + S), vzm = V(pb - S);
     R du;
-    if (ANISO) {
+    if (FLAT) {
+        const R vxx = (vxm + vxp - R(2) * vc) * A.inv_h2;
+        const R vyy = (vym + vyp - R(2) * vc) * A.inv_h2;
+        if (ANISO) {
+            const R vxy = (V(pb + sx + sy) + V(pb - sx - sy) - V(pb - sx + sy) - V(pb + sx - sy)) * A.inv_4h2;
+            const uint32_t ok = k * S;
+            const R f0 = ld_nt(A.fib, ok), f1 = ld_nt(A.fib + A.pitch, ok);
+            const R b0 = ld_nt(A.drift, ok), b1 = ld_nt(A.drift + A.pitch, ok);
+            const R dd = A.ddiff, dc = A.dcross;
+            du = (dc + dd * f0 * f0) * vxx + (dc + dd * f1 * f1) * vyy + R(2) * dd * f0 * f1 * vxy +
+                 (b0 * (vxp - vxm) + b1 * (vyp - vym)) * A.inv_2h;
+        } else {
+            du = A.dcross * (vxx + vyy);
+        }
+    } else if (ANISO) {
+        const R vzp = V(pb + S), vzm = V(pb - S);
         const R vxx = (vxm + vxp - R(2) * vc) * A.inv_h2;
         const R vyy = (vym + vyp - R(2) * vc) * A.inv_h2;
         const R vzz = (vzm + vzp - R(2) * vc) * A.inv_h2;
@@ -62,6 +78,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LQT_TISSUE_
              R(2) * dd * (f0 * f1 * vxy + f0 * f2 * vxz + f1 * f2 * vyz) +
              (b0 * (vxp - vxm) + b1 * (vyp - vym) + b2 * (vzp - vzm)) * A.inv_2h;
     } else {
+        const R vzp = V(pb + S), vzm = V(pb - S);
         du = A.dcross * ((vxm + vxp - R(2) * vc) + (vym + vyp - R(2) * vc) + (vzm + vzp - R(2) * vc)) * A.inv_h2;
     }
 
@@ -116,18 +133,22 @@ void launch_copy(void* dst, const void* src, int64_t bytes, hipStream_t st) {
     copy_kernel<<<(n + B - 1) / B, B, 0, st>>>((float4*)dst, (const float4*)src, n);
 }
 
-void launch_tissue_step(bool fp32, bool aniso, const void* args, int64_t nvox, hipStream_t st) {
+template <class R>
+static void launch_t(bool aniso, bool flat, const TissueArgs<R>& a, int64_t G, int B, hipStream_t st) {
+    if (flat) {
+        if (aniso) tissue_step_kernel<R, true, true><<<G, B, 0, st>>>(a);
+        else tissue_step_kernel<R, false, true><<<G, B, 0, st>>>(a);
+    } else {
+        if (aniso) tissue_step_kernel<R, true, false><<<G, B, 0, st>>>(a);
+        else tissue_step_kernel<R, false, false><<<G, B, 0, st>>>(a);
+    }
+}
+
+void launch_tissue_step(bool fp32, bool aniso, bool flat, const void* args, int64_t nvox, hipStream_t st) {
     const int B = 256;
     const int64_t G = (nvox + B - 1) / B;
-    if (fp32) {
-        const TissueArgs<float>& a = *(const TissueArgs<float>*)args;
-        if (aniso) tissue_step_kernel<float, true><<<G, B, 0, st>>>(a);
-        else tissue_step_kernel<float, false><<<G, B, 0, st>>>(a);
-    } else {
-        const TissueArgs<double>& a = *(const TissueArgs<double>*)args;
-        if (aniso) tissue_step_kernel<double, true><<<G, B, 0, st>>>(a);
-        else tissue_step_kernel<double, false><<<G, B, 0, st>>>(a);
-    }
+    if (fp32) launch_t(aniso, flat, *(const TissueArgs<float>*)args, G, B, st);
+    else launch_t(aniso, flat, *(const TissueArgs<double>*)args, G, B, st);
 }
 
 void launch_ghost_fill(bool fp32, void* grid, const int32_t* dst, const int32_t* src, int64_t n,

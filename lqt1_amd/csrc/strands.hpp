@@ -11,6 +11,9 @@ struct StrandDev {
     double dx, bcl, stop_time, stim_start, stim_dur, stim_strength, rec_from, x0;
     int64_t phi_off, phi_cap;
     int64_t vrec_off, vrec_cap;  // rows
+    // S2 (OneD_VW.cc:291-301): only while the S1 window is closed
+    double s2_base, s2_start, s2_dur, s2_strength;
+    int s2_first, s2_count;
 };
 
 // Per-cell beat tracker: -30 mV activation (OneD.cc:422), running peak,
@@ -45,6 +48,7 @@ struct StrandsArgs {
     SoA<R> fin;                // final state, pitch = total cells
     R* finv;
     int64_t* steps;            // per strand step count
+    double* vmax;              // per strand [2]: peak V of cell 0 / cell n-1 once time - s2_base >= s2_start
 };
 
 void launch_strands(bool fp32, int block, int grid, const void* args, hipStream_t st);

@@ -28,6 +28,14 @@ class StrandSpec:
     measure_beat: int | None = None  # default the last beat
     rec_all_beats: bool = False
     vrec_every: int = 0             # V of every cell every vrec_every steps (0 = off)
+    rec_from: float | None = None   # default: start of the measured beat (0 with rec_all_beats)
+    # S2 (OneD_VW.cc:291-301): time - s2_base in [s2_start, s2_start + s2_duration)
+    s2_start: float = 0.0
+    s2_duration: float = 0.0        # 0 = no S2
+    s2_strength: float = -104.0
+    s2_first: int = 10
+    s2_count: int = 3
+    s2_base: float | None = None    # default (nbeats - 1) * bcl
     extra: dict = field(default_factory=dict)
 
 
@@ -53,10 +61,12 @@ class Strands:
             L_ = ct.size
             stop = s.stop_time if s.stop_time is not None else s.nbeats * s.bcl
             mb = s.measure_beat if s.measure_beat is not None else int(stop // s.bcl) - 1
-            rec_from = 0.0 if s.rec_all_beats else mb * s.bcl
+            rec_from = s.rec_from if s.rec_from is not None else (0.0 if s.rec_all_beats else mb * s.bcl)
             x0 = L_ * s.dx + s.electrode_mm
+            s2b = s.s2_base if s.s2_base is not None else (s.nbeats - 1) * s.bcl
             descs[i] = L.StrandDesc(L_, off, i, s.dx, s.bcl, stop, s.stim_start, s.stim_duration,
-                                    s.stim_strength, s.nstim, s.rec_every, rec_from, x0, mb, s.vrec_every)
+                                    s.stim_strength, s.nstim, s.rec_every, rec_from, x0, mb, s.vrec_every,
+                                    s2b, s.s2_start, s.s2_duration, s.s2_strength, s.s2_first, s.s2_count)
             cts.append(ct)
             coeffs.append(np.broadcast_to(np.asarray(s.coeff, np.float64), (L_,)))
             plist.append(s.params)
@@ -100,7 +110,8 @@ class Strands:
         out = (L.StrandResult * self.n)()
         L.check(L.lib().lqt_strands_results(self.h, out))
         return [dict(qt=r.qt, apd90=tuple(r.apd90), ead=r.ead, act_first=r.act_first,
-                     act_last=r.act_last, cv=r.cv, steps=r.steps) for r in out]
+                     act_last=r.act_last, cv=r.cv, steps=r.steps, vmax_first=r.vmax_first,
+                     vmax_last=r.vmax_last) for r in out]
 
     def cell_results(self, i):
         n = self._descs[i].ncell

@@ -50,6 +50,11 @@ class Track(C.Structure):
                 ("vmax", C.c_double), ("vmin", C.c_double), ("flags", C.c_int)]
 
 
+class S2(C.Structure):
+    _fields_ = [("base", C.c_double), ("start", C.c_double), ("duration", C.c_double),
+                ("strength", C.c_double), ("first", C.c_int), ("count", C.c_int)]
+
+
 class Diag(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("Iion", "dV_dt", "ICaL", "IKs", "IKr", "IK1", "INa")]
 
@@ -92,6 +97,12 @@ def lib():
                                      C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int64,
                                      C.POINTER(C.c_int64), C.c_int, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p]
+        L.or_strand_vw.restype = C.c_int64
+        L.or_strand_vw.argtypes = [C.c_int, u8, dp, C.POINTER(Params), C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_int, C.POINTER(S2), dp, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int64), C.c_double, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_int, C.c_int, C.c_int64, C.POINTER(C.c_int64)]
         L.or_track_init.argtypes = [C.POINTER(Track)]
         L.or_track_step.argtypes = [C.POINTER(Track), C.c_double, C.c_double]
         L.or_track_result.restype = C.c_int
@@ -175,6 +186,76 @@ def strand_run(celltype, coeff, p=None, dx=0.15, dt=0.02, bcl=1000.0, stop_time=
     r = rows.value
     return dict(v=None if vrec is None else vrec[:r], t=None if rt is None else rt[:r],
                 act=act, apd=apd, ead=ead, state=fin, steps=int(steps))
+
+
+class StrandVW:
+    """OneD_VW.cc:239-398 for one S2 candidate, resumable (oracle restatement).
+
+    ``run(pause_at)`` advances to the first step with time >= pause_at (or
+    past the stop time); ``fork(s2_start)`` copies the paused run for
+    another candidate, which is exact because the S2 test is false for
+    every step before its window.  ``vmax`` holds the peaks of cell 0 and
+    cell n-1 (OneD_VW.cc:390-394), ``rows``/``row_t`` the recorded V."""
+
+    def __init__(self, celltype, coeff, p=None, dx=0.15, dt=0.02, bcl=1000.0, nbeats=5, stim=-52.0,
+                 stim_dur=3.0, stim_start=0.0, nstim=3, s2_start=260.0, s2_duration=3.0, s2_strength=-104.0,
+                 s2_first=10, s2_count=3, rec_every=50, record=True):
+        self.ct = np.ascontiguousarray(celltype, np.uint8)
+        n = self.ct.size
+        self.coeff = np.ascontiguousarray(np.broadcast_to(np.asarray(coeff, np.float64), (n,)))
+        self.p = p or params()
+        self.dx, self.dt, self.bcl, self.nbeats = dx, dt, bcl, nbeats
+        self.stim = (stim, stim_dur, stim_start, nstim)
+        self.s2 = S2((nbeats - 1) * bcl, s2_start, s2_duration, s2_strength, s2_first, s2_count)
+        self.state = np.zeros((n, NSTATE))
+        for i in range(n):
+            self.state[i] = cell_init()
+        self.time = C.c_double(0.0)
+        self.step = C.c_int64(0)
+        self.vmax = np.array([-100.0, -100.0])
+        self.rec_every = rec_every if record else 0
+        cap = int(bcl / dt / max(rec_every, 1)) + 2 if record else 0
+        self.rows = np.zeros((cap, n)) if record else None
+        self.row_t = np.zeros(cap) if record else None
+        self.nrows = C.c_int64(0)
+
+    def run(self, pause_at=np.inf):
+        st, sd, ss, ns = self.stim
+        lib().or_strand_vw(self.ct.size, self.ct, self.coeff, C.byref(self.p), self.dx, self.dt, self.bcl,
+                           self.nbeats * self.bcl, st, sd, ss, ns, C.byref(self.s2), self.state,
+                           C.byref(self.time), C.byref(self.step), float(pause_at), _ptr(self.vmax),
+                           _ptr(self.rows), _ptr(self.row_t), self.rec_every, self.nbeats - 1,
+                           0 if self.rows is None else self.rows.shape[0], C.byref(self.nrows))
+        return self
+
+    def fork(self, s2_start):
+        import copy
+        o = copy.copy(self)
+        o.state = self.state.copy()
+        o.time = C.c_double(self.time.value)
+        o.step = C.c_int64(self.step.value)
+        o.vmax = self.vmax.copy()
+        o.s2 = S2(self.s2.base, s2_start, self.s2.duration, self.s2.strength, self.s2.first, self.s2.count)
+        o.rows = None if self.rows is None else self.rows.copy()
+        o.row_t = None if self.row_t is None else self.row_t.copy()
+        o.nrows = C.c_int64(self.nrows.value)
+        return o
+
+    def v_rows(self):
+        r = self.nrows.value
+        return self.row_t[:r], self.rows[:r]
+
+
+def vw_candidates(left=260.0, step=0.1, count=1, direction=1):
+    """The S2 times OneD_VW.cc's MODE2 (direction +1, :405-423) or MODE3
+    (-1, :427-443) visits: s2 = left, then left += step (or -=) each time,
+    accumulated in double as the reference does."""
+    out = []
+    s = left
+    for _ in range(count):
+        out.append(s)
+        s = s + step if direction > 0 else s - step
+    return np.array(out)
 
 
 def track_trace(t, v):

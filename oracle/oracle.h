@@ -10,6 +10,7 @@
  *   SingleCell/TPORd.cc:379-1021  (update)    -> or_cell_update
  *   SingleCell.cc:95-232          (protocol)  -> or_single_cell
  *   OneD.cc:100-131,315-436       (strand)    -> or_strand_run
+ *   OneD_VW.cc:239-398            (S2 candidate) -> or_strand_vw
  *   GPU3D.cu:258-308,370-385,1222-1230, new_11.cu:352-417 (tissue stencil)
  *                                             -> or_tissue_*
  *   GPU3D.cu:1487-1680            (ghost map) -> or_ghost_map
@@ -113,13 +114,39 @@ int64_t or_strand_run2(int n, const uint8_t *celltype, const double *coeff,
                        int64_t *nrows, int act_beat, double *act, double *apd, int *ead,
                        double *final_state);
 
+/* S2 of OneD_VW.cc:291-301: while the S1 window is closed and
+ * time - base lies in [start, start + duration), cells first ..
+ * first+count-1 get `strength` (the reference: base (numS1-1)*BCL, start the
+ * candidate s2startTime, duration 3 ms, strength -104, cells 10..12). */
+typedef struct {
+    double base, start, duration, strength;
+    int first, count;
+} or_s2;
+
+/* OneD_VW.cc:239-398 for one S2 candidate, resumable: advances the caller's
+ * state s[n*40] from (*time, *step) while time <= stop_time and
+ * time < pause_at, and leaves (*time, *step) at the first step not run.
+ * vmax[2] (in/out; the reference starts both at -100) takes the peak V of
+ * cell 0 and cell n-1 after each update whose time - base >= start
+ * (OneD_VW.cc:390-394).  V rows are recorded as OneD_VW.cc:370-385 writes
+ * them: steps with step % rec_every == 0 inside beat rec_beat
+ * (floor(time/BCL) == rec_beat; -1 = every beat).  Returns *step. */
+int64_t or_strand_vw(int n, const uint8_t *celltype, const double *coeff,
+                     const or_params *p, double dx, double dt, double bcl,
+                     double stop_time, double stim_strength, double stim_dur,
+                     double stim_start, int nstim, const or_s2 *s2,
+                     double *s, double *time, int64_t *step, double pause_at,
+                     double *vmax, double *vrec, double *rec_t, int rec_every,
+                     int rec_beat, int64_t rec_cap, int64_t *nrows);
+
 /* Pseudo-ECG of PlotScript/plot1DECG_compare.py:89-105 over T rows of L cells. */
 void or_pseudo_ecg(int64_t T, int L, const double *v, double dx, double x0, double *phi);
 
 /* ---- tissue ------------------------------------------------------------ */
 
 /* Ghost map of GPU3D.cu:1301-1333 and 1487-1604 on an nx*ny*nz grid with
- * x slowest and z fastest (index = (x*ny + y)*nz + z).  mask[c] != 0 marks
+ * x slowest and z fastest (index = (x*ny + y)*nz + z); with nz == 1 the 2D
+ * sheet rule of new_11.cu:1535-1604 (8 neighbours).  mask[c] != 0 marks
  * tissue.  On return g[c] = k+1 for the k-th tissue voxel (x,y,z order),
  * -(k+1) for a ghost cell mirroring tissue voxel k, 0 otherwise.
  * Returns the number of tissue voxels. */
@@ -134,7 +161,8 @@ void or_tissue_tensor(int nx, int ny, int nz, const int32_t *g, int64_t nvox,
 
 /* du[k] = div(D grad V) at tissue voxel k with the 19-point stencil; V of a
  * neighbour is read through g (tissue, ghost mirror, or -90 mV outside,
- * GPU3D.cu:258-308). */
+ * GPU3D.cu:258-308).  With nz == 1 the sheet is 2D and the z terms are
+ * absent (new_11.cu:352-417, 9-point). */
 void or_tissue_du(int nx, int ny, int nz, const int32_t *g, int64_t nvox,
                   const double *dten, const double *drift, double dx,
                   const double *v, double *du);

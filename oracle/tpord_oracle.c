@@ -688,6 +688,83 @@ int64_t or_strand_run2(int n, const uint8_t *celltype, const double *coeff,
     return step;
 }
 
+/* OneD_VW.cc:239-398, one S2 candidate.  The S1 test and the diffusion
+ * term are those of OneD.cc (OneD_VW.cc:264-279,306-338); S2 is the
+ * `else if` of :291-301, so it is looked at only while the S1 window is
+ * closed; the peaks are :390-394, taken after the update with the time of
+ * the step.  The caller owns the state so that a run can pause (at the
+ * first step with time >= pause_at) and resume, or be copied to fork
+ * candidates that share their S1 prefix. */
+int64_t or_strand_vw(int n, const uint8_t *celltype, const double *coeff,
+                     const or_params *p, double dx, double dt, double bcl,
+                     double stopTime, double stimStrength, double stimDuration,
+                     double stimStart, int nstim_cells, const or_s2 *s2,
+                     double *s, double *time_io, int64_t *step_io, double pause_at,
+                     double *vmax, double *vrec, double *rec_t, int rec_every,
+                     int rec_beat, int64_t rec_cap, int64_t *nrows)
+{
+    double *oldV = (double *)malloc(sizeof(double) * (size_t)n);
+    double *dcoeff = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        if (n == 1) dcoeff[i] = 0.0;
+        else if (i == 0) dcoeff[i] = (coeff[i + 1] - coeff[i]) / dx;
+        else if (i == n - 1) dcoeff[i] = (coeff[i] - coeff[i - 1]) / dx;
+        else dcoeff[i] = (coeff[i + 1] - coeff[i - 1]) / (2.0 * dx);
+    }
+    int64_t rows = nrows ? *nrows : 0;
+    double time = *time_io;
+    int64_t step = *step_io;
+    for (; time <= stopTime && time < pause_at; time += dt, step++) {
+        for (int i = 0; i < n; ++i) oldV[i] = s[(size_t)i * OR_NSTATE + OR_V];
+        int s1_on = (time - floor(time / bcl) * bcl >= stimStart &&
+                     time - floor(time / bcl) * bcl < stimStart + stimDuration);
+        int s2_on = !s1_on && s2 && time - s2->base >= s2->start &&
+                    time - s2->base < s2->start + s2->duration;
+#pragma omp parallel for schedule(static) if (n >= 512)
+        for (int i = 0; i < n; ++i) {
+            double ist = 0.0;
+            if (s1_on) {
+                if (i < nstim_cells) ist = stimStrength;
+            } else if (s2_on && i >= s2->first && i < s2->first + s2->count) {
+                ist = s2->strength;
+            }
+            double first_order, second_order;
+            if (n == 1) {
+                first_order = 0.0;
+                second_order = 0.0;
+            } else if (i == 0) {
+                first_order = (oldV[i + 1] - oldV[i]) / (1.0 * dx);
+                second_order = (oldV[i + 1] + oldV[i] - 2.0 * oldV[i]) / (dx * dx);
+            } else if (i < n - 1) {
+                first_order = (oldV[i + 1] - oldV[i - 1]) / (2.0 * dx);
+                second_order = (oldV[i + 1] + oldV[i - 1] - 2.0 * oldV[i]) / (dx * dx);
+            } else {
+                first_order = (oldV[i] - oldV[i - 1]) / (1.0 * dx);
+                second_order = (oldV[i] + oldV[i - 1] - 2.0 * oldV[i]) / (dx * dx);
+            }
+            double dVgap_dt = dcoeff[i] * first_order + coeff[i] * second_order;
+            or_cell_update(s + (size_t)i * OR_NSTATE, celltype[i], p, dt, ist, dVgap_dt, 0);
+        }
+        if (vrec && rec_every > 0 && (rec_beat < 0 || floor(time / bcl) == rec_beat) &&
+            step % rec_every == 0 && rows < rec_cap) {
+            for (int i = 0; i < n; ++i) vrec[rows * n + i] = s[(size_t)i * OR_NSTATE + OR_V];
+            if (rec_t) rec_t[rows] = time;
+            rows++;
+        }
+        if (vmax && s2 && time - s2->base >= s2->start) {
+            double ve = s[OR_V], vl = s[(size_t)(n - 1) * OR_NSTATE + OR_V];
+            if (vl >= vmax[1]) vmax[1] = vl;
+            if (ve >= vmax[0]) vmax[0] = ve;
+        }
+    }
+    *time_io = time;
+    *step_io = step;
+    if (nrows) *nrows = rows;
+    free(oldV);
+    free(dcoeff);
+    return step;
+}
+
 int64_t or_strand_run(int n, const uint8_t *celltype, const double *coeff,
                       const or_params *p, double dx, double dt, double bcl,
                       double stopTime, double stimStrength, double stimDuration,
@@ -723,12 +800,65 @@ void or_pseudo_ecg(int64_t T, int L, const double *v, double dx, double x0, doub
  * for an nx*ny*nz grid.  The 26 neighbours are numbered as in the
  * reference: gg[1..9] at x-1, gg[10..17] at x, gg[18..26] at x+1, each
  * block in (y, z) lexicographic order skipping the centre. */
+/* The 2D sheet of new_11.cu:1535-1604 (nz == 1): 8 neighbours numbered
+ *   | 1 2 3 |
+ *   | 4 s 5 |      (x to the right, y down)
+ *   | 6 7 8 |
+ * il (x) and ir (y) summed in the reference's order, the tissue neighbour
+ * nearest to (il, ir)/max(|il|,|ir|) searched y outer, x inner.  The
+ * reference reads past the grid edge for x+1 == X and y+1 == Y (its tests
+ * are `x+1 >= 0`, `y+1 >= 0`); here such neighbours count as non-tissue,
+ * which is what the reference sees whenever the tissue keeps a one-cell
+ * margin. */
+static void ghost_map_2d(int nx, int ny, int32_t *g)
+{
+    const double root2 = sqrt(2.0);
+    int gg[9];
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y) {
+            if (g[(int64_t)x * ny + y] != 0) continue;
+            static const int ox[9] = {0, -1, 0, 1, -1, 1, -1, 0, 1};
+            static const int oy[9] = {0, -1, -1, -1, 0, 0, 1, 1, 1};
+            int isbound = 0;
+            for (int i = 1; i <= 8; ++i) {
+                int xx = x + ox[i], yy = y + oy[i];
+                int val = (xx >= 0 && xx < nx && yy >= 0 && yy < ny) ? g[(int64_t)xx * ny + yy] : 0;
+                gg[i] = val > 0 ? 1 : 0;
+                isbound += gg[i];
+            }
+            if (isbound == 0) continue;
+            double il = -gg[1] / root2 + gg[3] / root2 - gg[4] + gg[5] - gg[6] / root2 + gg[8] / root2;
+            double ir = -gg[1] / root2 - gg[2] - gg[3] / root2 + gg[6] / root2 + gg[7] + gg[8] / root2;
+            double imax = fabs(il);
+            if (fabs(ir) > imax) imax = fabs(ir);
+            double nxv = il / imax, nyv = ir / imax;
+            if (imax - 0 < 0.0001) { nxv = 0; nyv = 0; }
+            double dislow = 1000;
+            int bx = 0, by = 0;
+            for (int yc = -1; yc <= 1; ++yc)
+                for (int xc = -1; xc <= 1; ++xc) {
+                    double dis = sqrt(1.0 * (xc - nxv) * (xc - nxv) + (yc - nyv) * (yc - nyv));
+                    int xx = x + xc, yy = y + yc;
+                    if (xx >= 0 && xx < nx && yy >= 0 && yy < ny)
+                        if (g[(int64_t)xx * ny + yy] > 0 && dis < dislow) {
+                            bx = xc; by = yc;
+                            dislow = dis;
+                        }
+                }
+            g[(int64_t)x * ny + y] = -g[(int64_t)(x + bx) * ny + (y + by)];
+        }
+}
+
 int64_t or_ghost_map(int nx, int ny, int nz, const uint8_t *mask, int32_t *g)
 {
     const int64_t DX = (int64_t)ny * nz, DY = nz;
     const int64_t ncell = (int64_t)nx * DX;
     int64_t num = 0;
     for (int64_t c = 0; c < ncell; ++c) g[c] = mask[c] ? (int32_t)(++num) : 0;
+    if (nz == 1) {
+        ghost_map_2d(nx, ny, g);
+        return num;
+    }
     const double root2 = sqrt(2.0), root3 = sqrt(3.0);
     int gg[27];
     for (int x = 0; x < nx; ++x)
@@ -879,19 +1009,24 @@ void or_tissue_du(int nx, int ny, int nz, const int32_t *g, int64_t nvox,
         double vc = v[k];
         double vxp = NB(1, 0, 0), vxm = NB(-1, 0, 0);
         double vyp = NB(0, 1, 0), vym = NB(0, -1, 0);
-        double vzp = NB(0, 0, 1), vzm = NB(0, 0, -1);
         double vxx = (vxm + vxp - 2 * vc) / h2;
         double vyy = (vym + vyp - 2 * vc) / h2;
-        double vzz = (vzm + vzp - 2 * vc) / h2;
         double vxy = (NB(1, 1, 0) + NB(-1, -1, 0) - NB(-1, 1, 0) - NB(1, -1, 0)) / (4 * h2);
-        double vxz = (NB(1, 0, 1) + NB(-1, 0, -1) - NB(-1, 0, 1) - NB(1, 0, -1)) / (4 * h2);
-        double vyz = (NB(0, 1, 1) + NB(0, -1, -1) - NB(0, -1, 1) - NB(0, 1, -1)) / (4 * h2);
         double vx = (vxp - vxm) / (2 * dx);
         double vy = (vyp - vym) / (2 * dx);
-        double vz = (vzp - vzm) / (2 * dx);
-#undef NB
         const double *D = dten + 6 * k;
         const double *b = drift + 3 * k;
+        if (nz == 1) {
+            /* the 2D sheet of new_11.cu:352-417: no z terms */
+            du[k] = D[0] * vxx + D[1] * vyy + 2 * D[3] * vxy + b[0] * vx + b[1] * vy;
+            continue;
+        }
+        double vzp = NB(0, 0, 1), vzm = NB(0, 0, -1);
+        double vzz = (vzm + vzp - 2 * vc) / h2;
+        double vxz = (NB(1, 0, 1) + NB(-1, 0, -1) - NB(-1, 0, 1) - NB(1, 0, -1)) / (4 * h2);
+        double vyz = (NB(0, 1, 1) + NB(0, -1, -1) - NB(0, -1, 1) - NB(0, 1, -1)) / (4 * h2);
+        double vz = (vzp - vzm) / (2 * dx);
+#undef NB
         du[k] = D[0] * vxx + D[1] * vyy + D[2] * vzz + 2 * (D[3] * vxy + D[4] * vxz + D[5] * vyz) +
                 b[0] * vx + b[1] * vy + b[2] * vz;
     }

@@ -18,6 +18,11 @@ Fixtures
                         V of 9 cells every 1 ms in the last beat, the time
                         column, per-cell -30 mV activation times of the last
                         beat from the 0.2 ms output, and the printed CV.
+  ref_oned_vw.npz       OneD_VW.cc as shipped (MODE2 from s2 = 260 ms): the
+                        printed maxEndo/maxEpi of each candidate, whether it
+                        printed "VM found!", and 16 columns of each
+                        candidate's Transmural1D_s2@<s2>.dat (V every 1 ms of
+                        the last beat).
 """
 from __future__ import annotations
 
@@ -39,6 +44,7 @@ VARIANTS = ("con_iso", "lqt1_iso", "suprep_iso", "lqt1_suprep_iso", "clamp")
 CELLTYPES = (O.ENDO, O.MCELL, O.EPI)
 CHECKPOINTS = (1, 10, 100, 1000, 5000)
 ONED_CELLS = (0, 12, 24, 25, 42, 59, 60, 80, 99)
+VW_CELLS = (0, 5, 10, 11, 12, 20, 24, 25, 40, 59, 60, 62, 63, 64, 80, 99)
 
 
 def cell_states():
@@ -105,10 +111,32 @@ def oned():
                         phi_rows_t=t[beat], phi=O.pseudo_ecg(v[beat]))
 
 
+def oned_vw():
+    """OneD_VW.cc as shipped (CON+ISO1, MODE2 from s2 = 260 ms): the printed
+    peaks of every candidate it runs, its verdict, and the output file of
+    each candidate at the VW_CELLS columns."""
+    exe = os.path.join(O.REF_DIR, "model_oned_vw")
+    with tempfile.TemporaryDirectory() as d:
+        os.mkdir(os.path.join(d, "Outputs"))
+        r = subprocess.run([exe], cwd=d, capture_output=True, text=True, check=True, timeout=3600)
+        names = sorted(f for f in os.listdir(os.path.join(d, "Outputs")) if f.startswith("Transmural1D_s2@"))
+        files = {n: np.loadtxt(os.path.join(d, "Outputs", n)) for n in names}
+    endo = [float(x) for x in re.findall(r"maxEndo = ([-0-9.eE+]+)", r.stdout)]
+    epi = [float(x) for x in re.findall(r"maxEpi = ([-0-9.eE+]+)", r.stdout)]
+    s2 = [float(re.search(r"@([0-9.]+)\.dat", n).group(1)) for n in names]
+    out = dict(max_endo=np.array(endo), max_epi=np.array(epi), found=int("VM found!" in r.stdout),
+               s2=np.array(s2), cells=np.array(VW_CELLS))
+    for k, n in enumerate(names):
+        out[f"v{k}"] = files[n][:, VW_CELLS]
+        out[f"shape{k}"] = np.array(files[n].shape)
+    np.savez_compressed(os.path.join(HERE, "ref_oned_vw.npz"), **out)
+
+
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("oracle/_ref is not built (needs the reference tree): run make -C oracle")
     cell_states()
     single_cell()
     oned()
+    oned_vw()
     print("fixtures written to", HERE)

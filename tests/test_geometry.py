@@ -78,3 +78,53 @@ def test_label_by_depth_split():
     assert (lab == A.ENDO).mean() == pytest.approx(0.30, abs=0.002)
     assert (lab == A.MCELL).mean() == pytest.approx(0.35, abs=0.002)
     assert (lab == A.EPI).mean() == pytest.approx(0.35, abs=0.002)
+
+
+def test_sheet_mirrors_known_answers_2d():
+    """The 2D rule of new_11.cu:1535-1604 on a rectangle: edge ghosts
+    mirror the adjacent voxel, corner ghosts the corner voxel diagonally."""
+    m = np.zeros((7, 6, 1), bool)
+    m[1:6, 1:5, 0] = True
+    for g in (A.ghost_map(_lab(m))[0], O.ghost_map(m.astype(np.uint8))[0]):
+        idx = lambda x, y: g[x, y, 0]
+        assert idx(0, 2) == -idx(1, 2)
+        assert idx(6, 3) == -idx(5, 3)
+        assert idx(3, 0) == -idx(3, 1)
+        assert idx(3, 5) == -idx(3, 4)
+        assert idx(0, 1) == -idx(1, 1)   # next to the corner: normal (1, 0.41) -> (1, 0)
+        assert idx(0, 0) == -idx(1, 1)   # corner
+        assert idx(6, 5) == -idx(5, 4)
+    # a diagonal edge: ghosts above the line y = x mirror along the diagonal
+    # or an axis, never into the ghost layer
+    t = np.zeros((10, 10, 1), bool)
+    for x in range(10):
+        t[x, :x, 0] = True
+    g, _, _ = A.ghost_map(_lab(t))
+    for x in range(10):
+        for y in range(x, min(x + 1, 10)):
+            if g[x, y, 0] < 0:
+                k = -g[x, y, 0]
+                px, py, _ = np.argwhere(g == k)[0]
+                assert max(abs(px - x), abs(py - y)) == 1 and py < px
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_library_map_equals_oracle_2d(seed):
+    rng = np.random.default_rng(seed)
+    shape = (41, 37, 1)
+    x, y = np.meshgrid(np.linspace(-1, 1, shape[0]), np.linspace(-1, 1, shape[1]), indexing="ij")
+    r = np.hypot(x, y / 0.9)
+    m = ((r < 0.9) & (r > 0.3 + 0.1 * rng.random()) & (rng.random(shape[:2]) > 0.07))[:, :, None]
+    g_lib, nv, ng = A.ghost_map(_lab(m))
+    g_or, nv2 = O.ghost_map(m.astype(np.uint8))
+    assert nv == nv2
+    assert np.array_equal(g_lib, g_or)
+    assert ng == int((g_or < 0).sum())
+
+
+def test_every_sheet_stencil_neighbour_is_tissue_or_ghost_2d():
+    geo = G.annulus_2d(n=120, r_out=8.0, wall=4.0)
+    g, _, _ = A.ghost_map(geo.label)
+    for p in np.argwhere(geo.label[:, :, 0] != A.NONE):
+        for o in ((1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (-1, -1), (1, -1), (-1, 1)):
+            assert g[p[0] + o[0], p[1] + o[1], 0] != 0

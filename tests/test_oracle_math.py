@@ -249,3 +249,45 @@ def test_discrete_laplacian_eigenmode():
         du = O.tissue_du(g, nvox, dten, drift, dx, v)
         lam = -4 * D / dx ** 2 * np.sin(k * dx / 2) ** 2
         assert np.allclose(du, lam * v, atol=1e-10)
+
+
+def test_sheet_stencil_exact_on_quadratics_2d():
+    """nz == 1 is new_11.cu's 2D sheet: on a quadratic V the 9-point
+    operator gives Dxx A_xx + Dyy A_yy + 2 Dxy A_xy (no z terms even though
+    Dzz = Dc), where a 3D reading of the same grid would add the -90 mV
+    off-grid z neighbours."""
+    mask = np.zeros((13, 11, 1), np.uint8)
+    mask[1:-1, 1:-1, 0] = 1
+    g, nvox = O.ghost_map(mask)
+    dx = 0.15
+    f = np.array([0.8, -0.6, 0.0])
+    fib = np.tile(f.astype(np.float32), (nvox, 1))
+    Df, Dc = 0.154, 0.0385
+    dten, drift = O.tissue_tensor(g, nvox, fib, Df, Dc, dx)
+    D = Dc * np.eye(2) + (Df - Dc) * np.outer(f[:2], f[:2])
+    A = np.array([[1.5, -0.4], [-0.4, 0.6]])
+    pos = np.argwhere(g > 0)[:, :2] * dx
+    v = 0.5 * np.einsum("ni,ij,nj->n", pos, A, pos) + pos @ np.array([0.3, -0.1])
+    du = O.tissue_du(g, nvox, dten, drift, dx, v)
+    idx = np.argwhere(g > 0)
+    keep = np.all((idx[:, :2] >= 3) & (idx[:, :2] <= np.array(g.shape[:2]) - 4), axis=1)
+    assert keep.sum() > 10
+    assert np.allclose(du[keep], np.sum(D * A), atol=1e-9)
+    # a uniform V is a fixed point everywhere, boundary included
+    du0 = O.tissue_du(g, nvox, dten, drift, dx, np.full(nvox, -80.0))
+    assert np.max(np.abs(du0)) < 1e-9
+
+
+def test_sheet_isotropic_conserves_charge_2d():
+    """Isotropic D on a 2D sheet with straight mirrored edges: the sum of du
+    over the tissue vanishes (no-flux boundary)."""
+    mask = np.zeros((12, 9, 1), np.uint8)
+    mask[1:-1, 1:-1, 0] = 1
+    g, nvox = O.ghost_map(mask)
+    fib = np.zeros((nvox, 3), np.float32)
+    dten, drift = O.tissue_tensor(g, nvox, fib, 0.154, 0.154, 0.15)
+    v = np.random.default_rng(1).uniform(-90, 40, nvox)
+    du = O.tissue_du(g, nvox, dten, drift, 0.15, v)
+    # the corner ghosts mirror the corner voxel diagonally: their flux does
+    # not enter the 5-point sum, so the telescoping is exact
+    assert abs(du.sum()) < 1e-9 * np.abs(du).sum()
