@@ -26,6 +26,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# rocprofv3 --pmc summary of the FP32 tissue kernel on this workload
+PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "tissue_step_ventricle3d.md")
 
 
 def log(*a):
@@ -73,6 +75,21 @@ def cpu_baseline(geo, steps, threads):
         O.tissue_step(g, nvox, dten, drift, geo.dx, 0.02, ct, p, s, ist)
     wall = time.perf_counter() - t0
     return nvox * steps / wall, wall
+
+
+def measured_traffic():
+    """HBM bytes per launch of the FP32 tissue kernel on this workload as the
+    committed rocprofv3 --pmc summary gives them (FETCH_SIZE and WRITE_SIZE
+    per dispatch, each scaled by what the counter reads for a copy kernel of
+    known size); None if the summary is absent."""
+    try:
+        with open(PROFILE_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1].split()[0])
+    except OSError:
+        pass
+    return None
 
 
 def strand_accuracy(A):
@@ -180,8 +197,10 @@ def main():
     except Exception as e:  # reported, not hidden
         acc["strand_error"] = str(e)
     achieved = r32["bytes"] / (r32["us_main"] * 1e-6) / 1e9
+    traffic = measured_traffic()
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                traffic_over_algorithmic=round(traffic / r32["bytes"], 4) if traffic else None,
                 kernel="tissue_step_kernel<float, true>", kernel_us=round(r32["us_main"], 2),
                 bytes_per_launch=r32["bytes"])
 

@@ -29,8 +29,11 @@ typedef int lqt_status;
 
 typedef enum { LQT_FP64 = 0, LQT_FP32 = 1 } lqt_precision;
 
-/* enum CellType of SingleCell/Cell.cc:20 (values kept). */
-enum { LQT_LVEPI = 0, LQT_LVENDO = 1, LQT_EPI = 2, LQT_ENDO = 3, LQT_MCELL = 4, LQT_NONE = 255 };
+/* enum CellType of SingleCell/Cell.cc:20 (values kept).  In a tissue label
+ * grid, LQT_HOLD may be OR'ed onto a type: the voxel is updated as that
+ * type and its V is then reset to -100 mV every step (the type-0 voxels of
+ * new_11.cu:1247-1248); such voxels are left out of the V guard's counts. */
+enum { LQT_LVEPI = 0, LQT_LVENDO = 1, LQT_EPI = 2, LQT_ENDO = 3, LQT_MCELL = 4, LQT_HOLD = 0x80, LQT_NONE = 255 };
 
 /* Runtime form of the reference's compile-time model flags
  * (SingleCell/TPORd.cc:30-36,45; GPU3D.cu:18-22). */

@@ -77,11 +77,12 @@ def build(force: bool = False, jobs: int = 8) -> str:
 
 
 DRIVERS = {"lqt_single_cell": "single_cell.cpp", "lqt_oned": "one_d.cpp", "lqt_oned_vw": "one_d_vw.cpp",
-           "lqt_tissue3d": "tissue3d.cpp"}
+           "lqt_tissue2d": "tissue2d.cpp", "lqt_tissue3d": "tissue3d.cpp"}
 
 
 def build_drivers() -> None:
-    """The C++ programs mirroring SingleCell.cc, OneD.cc and GPU3D.cu, linked
+    """The C++ programs mirroring SingleCell.cc, OneD.cc, OneD_VW.cc, new_11.cu
+    and GPU3D.cu, linked
     against liblqt.so (found next to them through the rpath)."""
     bindir = os.path.join(HERE, "bin")
     os.makedirs(bindir, exist_ok=True)

@@ -283,7 +283,7 @@ lqt_status lqt_tissue_create(const lqt_geometry* geo, const lqt_params* params, 
         const int x = (int)(c / SX), y = (int)((c / SY) % t->ny), z = (int)(c % t->nz);
         t->h_vpos[k] = (int32_t)(((int64_t)(x + 1) * t->py + (y + 1)) * t->pz + (z + 1));
         t->h_label[k] = geo->label[c];
-        cl[k] = (uint8_t)cell_class(geo->label[c]);
+        cl[k] = (uint8_t)(cell_class(geo->label[c] & 0x7f) | (geo->label[c] & LQT_HOLD));
     }
     std::vector<int32_t> gsigned, gd, gs;
     ghost_map(t->nx, t->ny, t->nz, geo->label, gsigned, nullptr, nullptr);

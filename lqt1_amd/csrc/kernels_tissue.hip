@@ -91,13 +91,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LQT_TISSUE_
 
     CellState<R> s;
     load_state(s, A.st, k);
-    const int c = A.cls[k];
-    const R vn = tpord_update<R, false>(s, vc, c, *A.consts, A.dt, ist, du, nullptr);
+    const int c = A.cls[k];  // class, bit 7: V held at -100 mV (new_11.cu:1247-1248)
+    const bool held = c & 0x80;
+    R vn = tpord_update<R, false>(s, vc, c & 3, *A.consts, A.dt, ist, du, nullptr);
+    if (held) vn = R(-100);
     store_state(s, A.st, k);
     *(R*)((char*)A.vnew + pb) = vn;
 
     // V guard of GPU3D.cu:1796-1829, one atomic per wavefront.
-    const bool lo = vn < R(-90), hi = vn > R(200), bad = !(vn == vn) || !(vn - vn == R(0));
+    const bool lo = vn < R(-90) && !held, hi = vn > R(200), bad = !(vn == vn) || !(vn - vn == R(0));
     const unsigned long long mlo = __ballot(lo), mhi = __ballot(hi), mbad = __ballot(bad);
     if ((mlo | mhi | mbad) && (threadIdx.x & 63) == (__ffsll((long long)(mlo | mhi | mbad)) - 1)) {
         if (mlo) atomicAdd(&A.guard[0], (unsigned long long)__popcll(mlo));

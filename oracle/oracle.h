@@ -169,7 +169,9 @@ void or_tissue_du(int nx, int ny, int nz, const int32_t *g, int64_t nvox,
 
 /* One explicit step of the whole tissue: du from V^n, then the cell update
  * of every voxel (OpenMP over voxels).  s has OR_NSTATE doubles per voxel;
- * ist[k] is the stimulus current of voxel k for this step. */
+ * ist[k] is the stimulus current of voxel k for this step.  celltype[k]
+ * with bit 0x80 set: updated as celltype & 0x7f, then V = -100 mV (the
+ * type-0 voxels of new_11.cu:1247-1248). */
 void or_tissue_step(int nx, int ny, int nz, const int32_t *g, int64_t nvox,
                     const double *dten, const double *drift, double dx, double dt,
                     const uint8_t *celltype, const or_params *p,

@@ -1041,6 +1041,9 @@ void or_tissue_step(int nx, int ny, int nz, const int32_t *g, int64_t nvox,
     for (int64_t k = 0; k < nvox; ++k) vbuf[k] = s[(size_t)k * OR_NSTATE + OR_V];
     or_tissue_du(nx, ny, nz, g, nvox, dten, drift, dx, vbuf, du);
 #pragma omp parallel for schedule(static)
-    for (int64_t k = 0; k < nvox; ++k)
-        or_cell_update(s + (size_t)k * OR_NSTATE, celltype[k], p, dt, ist ? ist[k] : 0.0, du[k], 0);
+    for (int64_t k = 0; k < nvox; ++k) {
+        or_cell_update(s + (size_t)k * OR_NSTATE, celltype[k] & 0x7f, p, dt, ist ? ist[k] : 0.0, du[k], 0);
+        /* the type-0 voxels of new_11.cu:1247-1248 */
+        if (celltype[k] & 0x80) s[(size_t)k * OR_NSTATE + OR_V] = -100.0;
+    }
 }

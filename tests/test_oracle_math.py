@@ -291,3 +291,25 @@ def test_sheet_isotropic_conserves_charge_2d():
     # the corner ghosts mirror the corner voxel diagonally: their flux does
     # not enter the 5-point sum, so the telescoping is exact
     assert abs(du.sum()) < 1e-9 * np.abs(du).sum()
+
+
+def test_held_voxels_stay_at_minus_100():
+    """Label bit 0x80: updated as its type, then V reset to -100 mV (the
+    type-0 voxels of new_11.cu:1247-1248); its neighbours see -100."""
+    mask = np.zeros((7, 7, 1), np.uint8)
+    mask[1:-1, 1:-1, 0] = 1
+    g, nvox = O.ghost_map(mask)
+    fib = np.zeros((nvox, 3), np.float32)
+    dten, drift = O.tissue_tensor(g, nvox, fib, 0.154, 0.154, 0.2)
+    ct = np.full(nvox, O.ENDO, np.uint8)
+    ct[12] |= 0x80
+    s = np.tile(O.cell_init(), (nvox, 1))
+    free = s.copy()
+    ctf = np.full(nvox, O.ENDO, np.uint8)
+    for _ in range(20):
+        O.tissue_step(g, nvox, dten, drift, 0.2, 0.02, ct, O.params(), s, None)
+        O.tissue_step(g, nvox, dten, drift, 0.2, 0.02, ctf, O.params(), free, None)
+        assert s[12, 0] == -100.0
+    # the held voxel pulls its neighbours below the free-running tissue
+    assert np.all(s[np.arange(nvox) != 12, 0] <= free[np.arange(nvox) != 12, 0] + 1e-12)
+    assert np.min(s[np.arange(nvox) != 12, 0] - free[np.arange(nvox) != 12, 0]) < -1e-3
