@@ -34,9 +34,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def tissue_run(A, geo, prec, warmup, steps):
+def tissue_run(A, geo, prec, warmup, steps, tune_steps=20):
     T = A.Tissue(geo.label, geo.fibre, geo.dx, 0.154, 0.0385, A.preset("config3"), prec, 0.02)
     T.set_stimulus(geo.stim_sites, geo.stim_offsets, bcl=1000.0, duration=3.0, strength=-52.0)
+    # launch shape by the autotuner (state restored after each trial)
+    best, trial_us = T.autotune(tune_steps) if tune_steps > 0 else (T.variant(), [])
+    shapes = A.Tissue.variants()
     T.step(warmup)
     A.lib().lqt_device_synchronize()
     t0 = time.perf_counter()
@@ -44,7 +47,9 @@ def tissue_run(A, geo, prec, warmup, steps):
     wall = time.perf_counter() - t0
     us_main, us_ghost = T.kernel_us()
     out = dict(nvox=T.nvox, nghost=T.nghost, wall=wall, us_main=us_main, us_ghost=us_ghost,
-               bytes=T.bytes_per_step(), v=T.v(), guard=T.v_guard().tolist())
+               bytes=T.bytes_per_step(), v=T.v(), guard=T.v_guard().tolist(),
+               tune=dict(shapes=[list(x) for x in shapes], trial_us=[round(u, 1) for u in trial_us], best=best),
+               shape=shapes[best])
     T.close()
     return out
 
@@ -77,15 +82,17 @@ def cpu_baseline(geo, steps, threads):
     return nvox * steps / wall, wall
 
 
-def measured_traffic():
-    """HBM bytes per launch of the FP32 tissue kernel on this workload as the
-    committed rocprofv3 --pmc summary gives them (FETCH_SIZE and WRITE_SIZE
-    per dispatch, each scaled by what the counter reads for a copy kernel of
-    known size); None if the summary is absent."""
+def measured_traffic(shape):
+    """Bytes per launch the FP32 tissue kernel of launch shape (block, XCD
+    map, waves) moves beyond L2 on this workload, as the committed
+    rocprofv3 --pmc summary gives them (FETCH_SIZE and WRITE_SIZE per
+    dispatch, each scaled by what the counter reads for a copy kernel of
+    known size); None if the summary has no line for the shape."""
+    key = "traffic_bytes_per_launch %d %d %d:" % (shape[0], int(shape[1]), shape[2])
     try:
         with open(PROFILE_SUMMARY) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key):
                     return float(line.split(":", 1)[1].split()[0])
     except OSError:
         pass
@@ -147,11 +154,13 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=0, help="default: --steps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--tune-steps", type=int, default=20, help="autotuner trial steps per launch shape (0 = off)")
     ap.add_argument("--no-strands", action="store_true")
     ap.add_argument("--strand-beats", type=int, default=10, help="beats per strand of the 512-strand sweep")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--profile", choices=["fp32", "fp64", "copy"], default=None,
-                    help="short run of one kernel for rocprofv3 (no JSON line)")
+    ap.add_argument("--profile", choices=["fp32", "fp64", "copy", "cells"], default=None,
+                    help="short run of one kernel for rocprofv3 (no JSON line); cells = the ionic update "
+                         "alone, --steps updates per launch with the state in registers")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (one GPU per box, "
@@ -173,20 +182,38 @@ def main():
         us = A._lib.bench_copy(nbytes, max(1, args.steps))
         log(f"copy {nbytes} B: {us:.1f} us per copy, {2 * nbytes / us / 1e3:.0f} GB/s read+write")
         return
+    if args.profile == "cells":
+        # the ionic update alone on the ventricle's voxel count and type mix
+        n = 6356728
+        ct = np.array([A.ENDO, A.MCELL, A.EPI], np.uint8)[np.arange(n) * 3 // n]
+        c = A.Cells(ct, A.preset("config3"), A.FP32, 0.02)
+        c.step(max(1, args.warmup))
+        A.lib().lqt_device_synchronize()
+        t0 = time.perf_counter()
+        c.step(max(1, args.steps))
+        A.lib().lqt_device_synchronize()
+        dt_s = time.perf_counter() - t0
+        log(f"cells: {n} cells x {args.steps} updates in one launch: {dt_s * 1e3:.1f} ms, "
+            f"{n * args.steps / dt_s / 1e9:.2f} G cell-steps/s")
+        return
     t0 = time.perf_counter()
     geo = G.ventricle_3d(seed=args.seed)
     log(f"geometry {time.perf_counter() - t0:.1f} s, {geo.nvox} voxels, {geo.stim_sites.size} stimulus sites")
     if args.profile in ("fp32", "fp64"):
-        r = tissue_run(A, geo, A.FP32 if args.profile == "fp32" else A.FP64, args.warmup, args.steps)
-        log(f"{args.profile}: kernel {r['us_main']:.1f} us, ghost {r['us_ghost']:.1f} us, "
+        r = tissue_run(A, geo, A.FP32 if args.profile == "fp32" else A.FP64, args.warmup, args.steps,
+                       args.tune_steps)
+        log(f"{args.profile}: shape {r['shape']} (trials {r['tune']['trial_us']} us), "
+            f"kernel {r['us_main']:.1f} us, ghost {r['us_ghost']:.1f} us, "
             f"{r['bytes'] / (r['us_main'] * 1e-6) / 1e9:.0f} GB/s algorithmic, {r['bytes']:.0f} B per launch")
         return
 
-    r32 = tissue_run(A, geo, A.FP32, args.warmup, args.steps)
-    log(f"FP32: {r32['wall'] / args.steps * 1e3:.3f} ms/step, kernel {r32['us_main']:.1f} us, ghost {r32['us_ghost']:.1f} us")
+    r32 = tissue_run(A, geo, A.FP32, args.warmup, args.steps, args.tune_steps)
+    log(f"FP32: shape {r32['shape']} (trials {r32['tune']['trial_us']} us), {r32['wall'] / args.steps * 1e3:.3f} "
+        f"ms/step, kernel {r32['us_main']:.1f} us, ghost {r32['us_ghost']:.1f} us")
     k64 = args.fp64_steps or args.steps
-    r64 = tissue_run(A, geo, A.FP64, args.warmup, k64)
-    log(f"FP64: {r64['wall'] / k64 * 1e3:.3f} ms/step, kernel {r64['us_main']:.1f} us")
+    r64 = tissue_run(A, geo, A.FP64, args.warmup, k64, args.tune_steps)
+    log(f"FP64: shape {r64['shape']} (trials {r64['tune']['trial_us']} us), {r64['wall'] / k64 * 1e3:.3f} ms/step, "
+        f"kernel {r64['us_main']:.1f} us")
 
     nvox = r32["nvox"]
     v32 = nvox * args.steps / r32["wall"]
@@ -197,11 +224,17 @@ def main():
     except Exception as e:  # reported, not hidden
         acc["strand_error"] = str(e)
     achieved = r32["bytes"] / (r32["us_main"] * 1e-6) / 1e9
-    traffic = measured_traffic()
+    traffic = measured_traffic(r32["shape"])
+    # the practical streaming ceiling of this GPU: a plain 2 GiB copy
+    copy_us = A._lib.bench_copy(2 << 30, 20)
+    copy_gbs = 2 * (2 << 30) / (copy_us * 1e-6) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 traffic_over_algorithmic=round(traffic / r32["bytes"], 4) if traffic else None,
-                kernel="tissue_step_kernel<float, true>", kernel_us=round(r32["us_main"], 2),
+                copy_gbs=round(copy_gbs, 1), frac_of_copy=round(achieved / copy_gbs, 4),
+                kernel="tissue_step_kernel<float, true, false, %d, %s, %d>" % (
+                    r32["shape"][0], str(r32["shape"][1]).lower(), r32["shape"][2]),
+                kernel_us=round(r32["us_main"], 2), tune=r32["tune"],
                 bytes_per_launch=r32["bytes"])
 
     strands = None
@@ -241,7 +274,7 @@ def main():
                                "concentrations, anisotropic D_f 0.154/D_c 0.0385, IKs x0.2 ISO on, dt 0.02 ms",
                    "voxels": nvox, "ghost_cells": r32["nghost"], "device": dev},
         "fp64": {"value": v64, "unit": "cell-steps/s", "steps": k64, "ms_per_step": r64["wall"] / k64 * 1e3,
-                 "kernel_us": round(r64["us_main"], 2)},
+                 "kernel_us": round(r64["us_main"], 2), "tune": r64["tune"]},
         "fp32_over_fp64": v32 / v64,
         "accuracy": acc,
         "strands": strands,

@@ -175,6 +175,19 @@ lqt_status lqt_tissue_write_vtk(lqt_tissue *t, const char *path, int stride);
  * same geometry. */
 lqt_status lqt_tissue_save(lqt_tissue *t, const char *path);
 lqt_status lqt_tissue_load(lqt_tissue *t, const char *path);
+/* Launch shapes of the fused step kernel (workgroup size; whether
+ * workgroups take voxel ranges in the XCD-aware order; waves per SIMD the
+ * registers are allocated for).  Shape 0 is the default; a 2D sheet always
+ * runs shape 0. */
+lqt_status lqt_tissue_variants(int *count);
+lqt_status lqt_tissue_variant_info(int i, int *block, int *xcd_map, int *waves);
+lqt_status lqt_tissue_set_variant(lqt_tissue *t, int i);
+lqt_status lqt_tissue_get_variant(const lqt_tissue *t, int *i);
+/* Autotuner: runs every launch shape for trial_steps steps from the current
+ * state (state, V, guard and clock are restored after each trial, so the
+ * simulation is unchanged), keeps the fastest and returns it in *best;
+ * us[count] (may be NULL) receives each shape's mean kernel time. */
+lqt_status lqt_tissue_autotune(lqt_tissue *t, int trial_steps, int *best, double *us);
 /* Average device time of the fused step kernel over the last call of
  * lqt_tissue_step, in microseconds (HIP events on the tissue's stream). */
 lqt_status lqt_tissue_last_kernel_us(const lqt_tissue *t, double *us_step_kernel, double *us_ghost_kernel);

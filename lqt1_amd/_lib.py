@@ -155,6 +155,11 @@ def lib():
     _sig(L, "lqt_strands_block", [_VP, P(C.c_int)])
     _sig(L, "lqt_strands_autotune", [_VP, C.c_int, P(C.c_int), _VP])
     _sig(L, "lqt_strands_set_mode", [_VP, C.c_int])
+    _sig(L, "lqt_tissue_variants", [P(C.c_int)])
+    _sig(L, "lqt_tissue_variant_info", [C.c_int, P(C.c_int), P(C.c_int), P(C.c_int)])
+    _sig(L, "lqt_tissue_set_variant", [_VP, C.c_int])
+    _sig(L, "lqt_tissue_get_variant", [_VP, P(C.c_int)])
+    _sig(L, "lqt_tissue_autotune", [_VP, C.c_int, P(C.c_int), _VP])
     _sig(L, "lqt_vw_search", [P(StrandDesc), _VP, _VP, P(Params), C.c_int, C.c_double, P(VWDesc), P(VWResult),
                               _VP, _VP, C.c_int64])
     _sig(L, "lqt_mlp_create", [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, P(_VP)])

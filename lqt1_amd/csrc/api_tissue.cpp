@@ -50,6 +50,7 @@ struct lqt_tissue {
     hipStream_t stream = nullptr;
     std::vector<hipEvent_t> ev;
     double us_step = 0, us_ghost = 0;
+    int variant = 0;  // launch shape of the fused step (kTissueVariants)
 };
 
 static size_t rsz(const lqt_tissue* t) { return t->fp32 ? sizeof(float) : sizeof(double); }
@@ -512,7 +513,7 @@ lqt_status lqt_tissue_step_async(lqt_tissue* t, int64_t nsteps, void* stream) {
             a.s2_on = t->ns2 > 0 && t->proto.s2_duration > 0 && t->time >= t->proto.s2_time &&
                       t->time < t->proto.s2_time + t->proto.s2_duration;
             if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i], s));
-            launch_tissue_step(true, t->aniso, t->nz == 1, &a, t->nvox, s);
+            launch_tissue_step(true, t->aniso, t->nz == 1, t->variant, &a, t->nvox, s);
         } else {
             TissueArgs<double> a;
             fill_args(t, a);
@@ -522,7 +523,7 @@ lqt_status lqt_tissue_step_async(lqt_tissue* t, int64_t nsteps, void* stream) {
             a.s2_on = t->ns2 > 0 && t->proto.s2_duration > 0 && t->time >= t->proto.s2_time &&
                       t->time < t->proto.s2_time + t->proto.s2_duration;
             if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i], s));
-            launch_tissue_step(false, t->aniso, t->nz == 1, &a, t->nvox, s);
+            launch_tissue_step(false, t->aniso, t->nz == 1, t->variant, &a, t->nvox, s);
         }
         if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i + 1], s));
         LQT_HIP(hipGetLastError());
@@ -635,6 +636,93 @@ lqt_status lqt_tissue_v_guard(lqt_tissue* t, int64_t counts[3]) {
     LQT_HIP(hipMemcpy(g, t->guard, sizeof(g), hipMemcpyDeviceToHost));
     for (int i = 0; i < 3; ++i) counts[i] = (int64_t)g[i];
     return LQT_OK;
+}
+
+lqt_status lqt_tissue_variants(int* count) {
+    LQT_CHECK(count, LQT_ERR_ARG, "lqt_tissue_variants: null argument");
+    *count = kNumTissueVariants;
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_variant_info(int i, int* block, int* xcd_map, int* waves) {
+    LQT_CHECK(i >= 0 && i < kNumTissueVariants, LQT_ERR_ARG, "lqt_tissue_variant_info: no variant %d", i);
+    if (block) *block = kTissueVariants[i].block;
+    if (xcd_map) *xcd_map = kTissueVariants[i].xcd;
+    if (waves) *waves = kTissueVariants[i].waves;
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_set_variant(lqt_tissue* t, int i) {
+    LQT_CHECK(t && i >= 0 && i < kNumTissueVariants, LQT_ERR_ARG, "lqt_tissue_set_variant: bad argument");
+    t->variant = i;
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_get_variant(const lqt_tissue* t, int* i) {
+    LQT_CHECK(t && i, LQT_ERR_ARG, "lqt_tissue_get_variant: bad argument");
+    *i = t->variant;
+    return LQT_OK;
+}
+
+// Every launch shape runs trial_steps steps from the same state; the state,
+// V grids, guard counters and clock are copied aside first and put back
+// after each trial, so the simulation is left exactly where it was.
+lqt_status lqt_tissue_autotune(lqt_tissue* t, int trial_steps, int* best, double* us) {
+    LQT_CHECK(t && trial_steps > 0, LQT_ERR_ARG, "lqt_tissue_autotune: bad argument");
+    LQT_HIP(hipStreamSynchronize(t->stream));
+    const size_t rs = rsz(t);
+    const size_t nr = rs * NR * t->pitch, nd = sizeof(double) * ND * t->pitch, ng = rs * t->ncell_pad;
+    void *sr = nullptr, *sd = nullptr, *sg0 = nullptr, *sg1 = nullptr, *sgu = nullptr;
+    auto release = [&]() {
+        hipFree(sr);
+        hipFree(sd);
+        hipFree(sg0);
+        hipFree(sg1);
+        hipFree(sgu);
+    };
+    if (hipMalloc(&sr, nr) != hipSuccess || hipMalloc(&sd, nd) != hipSuccess || hipMalloc(&sg0, ng) != hipSuccess ||
+        hipMalloc(&sg1, ng) != hipSuccess || hipMalloc(&sgu, 3 * sizeof(unsigned long long)) != hipSuccess) {
+        release();
+        return set_error(LQT_ERR_HIP, "lqt_tissue_autotune: no device memory for the state snapshot");
+    }
+    const double time0 = t->time;
+    const int64_t step0 = t->step;
+    const int cur0 = t->cur, keep = t->variant;
+    auto copy_all = [&](bool save) -> hipError_t {
+        hipError_t e = hipSuccess;
+        void* pairs[5][2] = {{sr, t->r}, {sd, t->d}, {sg0, t->grid[0]}, {sg1, t->grid[1]}, {sgu, t->guard}};
+        const size_t sz[5] = {nr, nd, ng, ng, 3 * sizeof(unsigned long long)};
+        for (int i = 0; i < 5 && e == hipSuccess; ++i)
+            e = save ? hipMemcpy(pairs[i][0], pairs[i][1], sz[i], hipMemcpyDeviceToDevice)
+                     : hipMemcpy(pairs[i][1], pairs[i][0], sz[i], hipMemcpyDeviceToDevice);
+        return e;
+    };
+    if (copy_all(true) != hipSuccess) {
+        release();
+        return set_error(LQT_ERR_HIP, "lqt_tissue_autotune: snapshot copy failed");
+    }
+    double tbest = 1e300;
+    int ibest = keep;
+    lqt_status st = LQT_OK;
+    for (int v = 0; v < kNumTissueVariants && !st; ++v) {
+        t->variant = v;
+        st = lqt_tissue_step(t, 1);  // first launch of this shape, untimed
+        if (!st) st = lqt_tissue_step(t, trial_steps);
+        const double u = t->us_step;
+        if (us) us[v] = u;
+        if (!st && u < tbest) {
+            tbest = u;
+            ibest = v;
+        }
+        t->time = time0;
+        t->step = step0;
+        t->cur = cur0;
+        if (copy_all(false) != hipSuccess && !st) st = set_error(LQT_ERR_HIP, "lqt_tissue_autotune: restore failed");
+    }
+    release();
+    t->variant = st ? keep : ibest;
+    if (best) *best = t->variant;
+    return st;
 }
 
 lqt_status lqt_tissue_last_kernel_us(const lqt_tissue* t, double* us_step_kernel, double* us_ghost_kernel) {

@@ -68,7 +68,15 @@ void launch_cells_pace(bool fp32, int n, void* r, double* d, int64_t pitch, void
                        const PaceArgs& a, hipStream_t st);
 
 // kernels_tissue.hip
-void launch_tissue_step(bool fp32, bool aniso, bool flat, const void* args, int64_t nvox, hipStream_t st);
+// Launch shapes of the fused tissue step (block size, XCD-aware map, waves
+// per SIMD the registers are allocated for); entry 0 is the default.
+struct TissueVariant {
+    int block, xcd, waves;
+};
+constexpr int kNumTissueVariants = 6;
+extern const TissueVariant kTissueVariants[kNumTissueVariants];
+void launch_tissue_step(bool fp32, bool aniso, bool flat, int variant, const void* args, int64_t nvox,
+                        hipStream_t st);
 void launch_ghost_fill(bool fp32, void* grid, const int32_t* dst, const int32_t* src, int64_t n,
                        hipStream_t st);
 void launch_fill(bool fp32, void* p, int64_t n, double value, hipStream_t st);

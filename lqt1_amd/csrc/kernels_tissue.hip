@@ -24,19 +24,17 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
     return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
 }
 
-// Waves per SIMD the register allocator targets (2 holds the whole FP32
-// anisotropic state without spilling).
-#ifndef LQT_TISSUE_WAVES
-#define LQT_TISSUE_WAVES 2
-#endif
-
-// FLAT: a 2D sheet (nz == 1, new_11.cu:352-417), no z terms.
-template <class R, bool ANISO, bool FLAT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LQT_TISSUE_WAVES))) void tissue_step_kernel(
-    TissueArgs<R> A) {
+// FLAT: a 2D sheet (nz == 1, new_11.cu:352-417), no z terms.  B lanes per
+// workgroup; XCD: the workgroup -> voxel-range map of xcd_tile (otherwise
+// workgroup b takes voxels b*B .. b*B+B-1); W: waves per SIMD the register
+// allocator targets (2 holds the FP32 anisotropic state in 192 registers;
+// FP64 spills at 2 and not at 1).
+template <class R, bool ANISO, bool FLAT, int B, bool XCD, int W>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W))) void tissue_step_kernel(TissueArgs<R> A) {
     // 32-bit byte offsets from uniform bases let the loads share address
     // registers (voxel and padded-grid counts are bounded on the host)
-    const uint32_t k = (uint32_t)(xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
+    const int64_t blk = XCD ? xcd_tile(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+    const uint32_t k = (uint32_t)(blk * B + threadIdx.x);
     if (k >= A.nvox) return;
     constexpr uint32_t S = sizeof(R);
     const uint32_t pb = (uint32_t)A.vpos[k] * S;  // byte offset of the voxel in the grid
@@ -135,22 +133,42 @@ void launch_copy(void* dst, const void* src, int64_t bytes, hipStream_t st) {
     copy_kernel<<<(n + B - 1) / B, B, 0, st>>>((float4*)dst, (const float4*)src, n);
 }
 
-template <class R>
-static void launch_t(bool aniso, bool flat, const TissueArgs<R>& a, int64_t G, int B, hipStream_t st) {
-    if (flat) {
-        if (aniso) tissue_step_kernel<R, true, true><<<G, B, 0, st>>>(a);
-        else tissue_step_kernel<R, false, true><<<G, B, 0, st>>>(a);
-    } else {
-        if (aniso) tissue_step_kernel<R, true, false><<<G, B, 0, st>>>(a);
-        else tissue_step_kernel<R, false, false><<<G, B, 0, st>>>(a);
+const TissueVariant kTissueVariants[kNumTissueVariants] = {
+    {256, 1, 2}, {256, 0, 2}, {128, 1, 2}, {512, 1, 2}, {256, 1, 1}, {128, 1, 1},
+};
+
+template <class R, bool ANISO, bool FLAT, int B, bool XCD, int W>
+static void launch_v(const TissueArgs<R>& a, int64_t nvox, hipStream_t st) {
+    tissue_step_kernel<R, ANISO, FLAT, B, XCD, W><<<(nvox + B - 1) / B, B, 0, st>>>(a);
+}
+
+template <class R, bool ANISO>
+static void launch_a(bool flat, int variant, const TissueArgs<R>& a, int64_t nvox, hipStream_t st) {
+    if (flat) {  // 2D sheets run the default shape only
+        launch_v<R, ANISO, true, 256, true, 2>(a, nvox, st);
+        return;
+    }
+    switch (variant) {
+        case 1: launch_v<R, ANISO, false, 256, false, 2>(a, nvox, st); break;
+        case 2: launch_v<R, ANISO, false, 128, true, 2>(a, nvox, st); break;
+        case 3: launch_v<R, ANISO, false, 512, true, 2>(a, nvox, st); break;
+        case 4: launch_v<R, ANISO, false, 256, true, 1>(a, nvox, st); break;
+        case 5: launch_v<R, ANISO, false, 128, true, 1>(a, nvox, st); break;
+        default: launch_v<R, ANISO, false, 256, true, 2>(a, nvox, st); break;
     }
 }
 
-void launch_tissue_step(bool fp32, bool aniso, bool flat, const void* args, int64_t nvox, hipStream_t st) {
-    const int B = 256;
-    const int64_t G = (nvox + B - 1) / B;
-    if (fp32) launch_t(aniso, flat, *(const TissueArgs<float>*)args, G, B, st);
-    else launch_t(aniso, flat, *(const TissueArgs<double>*)args, G, B, st);
+void launch_tissue_step(bool fp32, bool aniso, bool flat, int variant, const void* args, int64_t nvox,
+                        hipStream_t st) {
+    if (fp32) {
+        const TissueArgs<float>& a = *(const TissueArgs<float>*)args;
+        if (aniso) launch_a<float, true>(flat, variant, a, nvox, st);
+        else launch_a<float, false>(flat, variant, a, nvox, st);
+    } else {
+        const TissueArgs<double>& a = *(const TissueArgs<double>*)args;
+        if (aniso) launch_a<double, true>(flat, variant, a, nvox, st);
+        else launch_a<double, false>(flat, variant, a, nvox, st);
+    }
 }
 
 void launch_ghost_fill(bool fp32, void* grid, const int32_t* dst, const int32_t* src, int64_t n,

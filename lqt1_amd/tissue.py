@@ -78,6 +78,35 @@ class Tissue:
         L.check(L.lib().lqt_tissue_bytes_per_step(self.h, C.byref(b)))
         return b.value
 
+    @staticmethod
+    def variants():
+        """Launch shapes of the fused step kernel: [(block, xcd_map, waves), ...]."""
+        n = C.c_int()
+        L.check(L.lib().lqt_tissue_variants(C.byref(n)))
+        out = []
+        for i in range(n.value):
+            b, x, w = C.c_int(), C.c_int(), C.c_int()
+            L.check(L.lib().lqt_tissue_variant_info(i, C.byref(b), C.byref(x), C.byref(w)))
+            out.append((b.value, bool(x.value), w.value))
+        return out
+
+    def autotune(self, trial_steps=20):
+        """Time every launch shape from the current state (which is restored)
+        and keep the fastest; returns (best index, [us per shape])."""
+        n = len(self.variants())
+        us = np.zeros(n)
+        best = C.c_int()
+        L.check(L.lib().lqt_tissue_autotune(self.h, int(trial_steps), C.byref(best), L.ptr(us)))
+        return best.value, us.tolist()
+
+    def set_variant(self, i):
+        L.check(L.lib().lqt_tissue_set_variant(self.h, int(i)))
+
+    def variant(self):
+        i = C.c_int()
+        L.check(L.lib().lqt_tissue_get_variant(self.h, C.byref(i)))
+        return i.value
+
     def write_vtk(self, path, stride=3):
         L.check(L.lib().lqt_tissue_write_vtk(self.h, str(path).encode(), int(stride)))
 

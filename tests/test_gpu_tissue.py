@@ -140,3 +140,37 @@ def test_checkpoint_and_vtk(lqt, tmp_path):
     txt = (tmp_path / "ap.vtk").read_text().split("\n")
     assert txt[4].startswith("DIMENSIONS 4 4 4")
     assert T.v_guard()[2] == 0
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_launch_shapes_agree_and_autotune(lqt, prec):
+    """Every launch shape of the fused step (block size, XCD map, register
+    budget) computes each voxel the same way: identical V and states.  The
+    autotuner leaves the simulation where it was and keeps the fastest."""
+    P = lqt.FP32 if prec == "fp32" else lqt.FP64
+    lab, fib, sites = _blob(3, (16, 14, 12))
+    shapes = lqt.Tissue.variants()
+    assert len(shapes) >= 4 and shapes[0] == (256, True, 2)
+    ref = None
+    for i in range(len(shapes)):
+        T = lqt.Tissue(lab, fib, 0.2, 0.154, 0.0385, lqt.preset("cpu_default"), P, 0.02)
+        T.set_stimulus(sites, None, strength=-80.0)
+        T.set_variant(i)
+        T.step(120)
+        out = (T.v(), T.state(T.nvox // 3))
+        if ref is None:
+            ref = out
+        else:
+            assert np.array_equal(out[0], ref[0]), shapes[i]
+            assert np.array_equal(out[1], ref[1]), shapes[i]
+        T.close()
+    T = lqt.Tissue(lab, fib, 0.2, 0.154, 0.0385, lqt.preset("cpu_default"), P, 0.02)
+    T.set_stimulus(sites, None, strength=-80.0)
+    T.step(60)
+    v60, clock = T.v(), T.time()
+    best, us = T.autotune(8)
+    print(f"{prec} shapes {shapes}: us {np.round(us, 1).tolist()}, best {best}")
+    assert T.variant() == best and us[best] == min(us) and all(u > 0 for u in us)
+    assert np.array_equal(T.v(), v60) and T.time() == clock
+    T.step(60)
+    assert np.array_equal(T.v(), ref[0])
