@@ -158,9 +158,10 @@ def main():
     ap.add_argument("--no-strands", action="store_true")
     ap.add_argument("--strand-beats", type=int, default=10, help="beats per strand of the 512-strand sweep")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--profile", choices=["fp32", "fp64", "copy", "cells"], default=None,
+    ap.add_argument("--profile", choices=["fp32", "fp64", "copy", "cells", "strands"], default=None,
                     help="short run of one kernel for rocprofv3 (no JSON line); cells = the ionic update "
-                         "alone, --steps updates per launch with the state in registers")
+                         "alone, --steps updates per launch with the state in registers; strands = the "
+                         "512-strand sweep of --strand-beats beats")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (one GPU per box, "
@@ -181,6 +182,10 @@ def main():
         nbytes = 2 << 30
         us = A._lib.bench_copy(nbytes, max(1, args.steps))
         log(f"copy {nbytes} B: {us:.1f} us per copy, {2 * nbytes / us / 1e3:.0f} GB/s read+write")
+        return
+    if args.profile == "strands":
+        r = strands_throughput(A, 512, args.strand_beats, args.seed)
+        log(f"strands: {r}")
         return
     if args.profile == "cells":
         # the ionic update alone on the ventricle's voxel count and type mix

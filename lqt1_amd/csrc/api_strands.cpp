@@ -318,7 +318,7 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
         for (int i = 0; i < nparams; ++i) v0[i] = params[i].v0;
         lqt_status e;
         if ((e = upload(&s->ddesc, s->hdev.data(), nstrands))) return e;
-        if ((e = upload(&s->order, ord.data(), nstrands))) return e;
+        if ((e = upload(&s->order, ord.data(), ord.size()))) return e;  // both modes' orders
         if ((e = upload(&s->queue, (const int*)nullptr, 8))) return e;
         if ((e = upload(&s->cls, cl.data(), ncells))) return e;
         if ((e = upload(&s->coeff, co.data(), ncells))) return e;

@@ -90,14 +90,25 @@ def test_strands_fp32_within_gates(lqt, oracle_runs):
 
 def test_launch_layouts_agree_and_autotune(lqt):
     """Both launch layouts (per-size classes in parallel branches, or one
-    kernel at the largest block) give identical per-strand results."""
+    kernel at the largest block) give identical per-strand results; each
+    layout runs on its own handle so that a strand a layout skipped would
+    show up as missing output, and every strand reports its full step count."""
     specs = _specs(lqt)
-    st = lqt.Strands(specs, lqt.FP32, 0.02)
     outs = []
     for mode in (0, 1):
-        st.set_mode(mode)
-        st.run()
-        outs.append([st.cell_results(i) for i in range(len(specs))] + [st.ecg(i)[1] for i in range(len(specs))])
+        h = lqt.Strands(specs, lqt.FP32, 0.02)
+        h.set_mode(mode)
+        h.run()
+        for i, r in enumerate(h.results()):
+            n = 0
+            t = 0.0
+            while t <= specs[i].nbeats * specs[i].bcl:
+                t += 0.02
+                n += 1
+            assert r["steps"] == n, (mode, i, r["steps"], n)
+        outs.append([h.cell_results(i) for i in range(len(specs))] + [h.ecg(i)[1] for i in range(len(specs))])
+        h.close()
+    st = lqt.Strands(specs, lqt.FP32, 0.02)
     for k, (a, b) in enumerate(zip(outs[0], outs[1])):
         if isinstance(a, tuple):
             for x, y in zip(a, b):
