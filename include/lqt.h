@@ -262,11 +262,14 @@ lqt_status lqt_strands_final_state(lqt_strands *s, int i, double *out);
 lqt_status lqt_strands_last_ms(const lqt_strands *s, double *ms);
 /* Largest work-group size of the active launch layout (64..512). */
 lqt_status lqt_strands_block(const lqt_strands *s, int *block);
-/* Launch layouts: mode 0 runs one persistent kernel per block-size class
- * (strands of 1..8 wavefronts, each class at its own block size, all classes
- * as parallel branches of one graph); mode 1 runs every strand at the
- * largest block.  The autotuner times both on the batch cut to trial_steps
- * steps and keeps the faster (ms[2] receives the trial times). */
+/* Launch layouts: mode 0 runs one strand per workgroup, one persistent
+ * kernel per block-size class (strands of 1..8 wavefronts, each class at its
+ * own block size, all classes as parallel branches of one graph); mode 1 runs
+ * one strand per workgroup at the largest block; mode 2 packs strands of
+ * similar step counts on whole wavefronts into 512-lane workgroups.  The
+ * autotuner times every layout on the batch with all stop times scaled so
+ * that the longest strand runs trial_steps steps, and keeps the fastest
+ * (ms[3] receives the trial times). */
 lqt_status lqt_strands_autotune(lqt_strands *s, int trial_steps, int *mode, double *ms);
 lqt_status lqt_strands_set_mode(lqt_strands *s, int mode);
 

@@ -16,31 +16,38 @@ int strands_occupancy(bool fp32, int block);
 }
 using namespace lqt;
 
+constexpr int NMODES = 3;
+
 struct lqt_strands {
     int n = 0;
     int64_t ncells = 0, pitch = 0;
     bool fp32 = true;
     double dt = 0.02;
     int block = 512;  // largest block of the active layout
-    // launch layouts: mode 0 = one persistent kernel per block-size class
-    // (strands of 1..8 wavefronts), mode 1 = one kernel at the largest block
+    // launch layouts (NMODES):
+    //   0 = one strand per workgroup, one persistent kernel per block-size
+    //       class (strands of 1..8 wavefronts) as parallel graph branches;
+    //   1 = one strand per workgroup, one kernel at the largest block;
+    //   2 = strands packed on whole wavefronts into 512-lane workgroups
+    //       (strands of similar step counts together), one kernel.
     struct Layout {
-        int block, grid, first, count;
+        int block, grid, first, count;  // bundles [first, first + count)
     };
-    std::vector<Layout> layout[2];
+    std::vector<Layout> layout[NMODES];
     int mode = 0;
     bool tuned = false;
-    double tune_ms[2] = {0, 0};
+    double tune_ms[NMODES] = {0, 0, 0};
     hipStream_t side[8] = {};
     hipEvent_t fork = nullptr, join[8] = {};
-    hipGraph_t graph_m[2] = {nullptr, nullptr};
-    hipGraphExec_t exec_m[2] = {nullptr, nullptr};
+    hipGraph_t graph_m[NMODES] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t exec_m[NMODES] = {nullptr, nullptr, nullptr};
     std::vector<lqt_strand_desc> desc;
     std::vector<uint8_t> label;  // per cell enum CellType
     std::vector<StrandDev> hdev;
+    std::vector<Bundle> hbund;   // the bundles of all modes
     // device
     StrandDev* ddesc = nullptr;
-    int* order = nullptr;  // [2][n]: the strand order of each mode
+    Bundle* bund = nullptr;
     int* queue = nullptr;  // one work counter per layout entry
     uint8_t* cls = nullptr;
     double* coeff = nullptr;
@@ -79,9 +86,9 @@ static size_t rsz(const lqt_strands* s) { return s->fp32 ? sizeof(float) : sizeo
 
 template <class R>
 static void fill_sargs(lqt_strands* s, StrandsArgs<R>& a) {
-    a.nstrands = s->n;
+    a.nbundles = 0;
+    a.bundles = s->bund;
     a.desc = s->ddesc;
-    a.order = s->order;
     a.queue = s->queue;
     a.cls = s->cls;
     a.coeff = s->coeff;
@@ -109,15 +116,15 @@ static lqt_status launch_one(lqt_strands* s, int mode, int li, hipStream_t st) {
     if (s->fp32) {
         StrandsArgs<float> a;
         fill_sargs(s, a);
-        a.nstrands = L.count;
-        a.order = s->order + (size_t)mode * s->n + L.first;
+        a.nbundles = L.count;
+        a.bundles = s->bund + L.first;
         a.queue = s->queue + li;
         launch_strands(true, L.block, L.grid, &a, st);
     } else {
         StrandsArgs<double> a;
         fill_sargs(s, a);
-        a.nstrands = L.count;
-        a.order = s->order + (size_t)mode * s->n + L.first;
+        a.nbundles = L.count;
+        a.bundles = s->bund + L.first;
         a.queue = s->queue + li;
         launch_strands(false, L.block, L.grid, &a, st);
     }
@@ -193,6 +200,102 @@ static double qt_from_ecg(const T* phi, int64_t n, double t0, double dts, double
         if (fabs(phi[j]) <= 0.05 * pk) return t - t_q;
     }
     return -1.0;
+}
+
+// Step counts of `for (time = 0; time <= stop; time += dt)` (OneD.cc:315)
+// for every stop time.  The accumulator runs through the same values for
+// every strand, so it is run once up to the largest stop and each count is
+// a binary search in it.
+static std::vector<int64_t> step_counts(const std::vector<double>& stops, double dt) {
+    double tmax = 0.0;
+    for (double x : stops) tmax = std::max(tmax, x);
+    std::vector<double> seq;  // time at the start of step k
+    double t = 0.0;
+    while (t <= tmax) {
+        seq.push_back(t);
+        t += dt;
+    }
+    std::vector<int64_t> out(stops.size());
+    for (size_t i = 0; i < stops.size(); ++i)
+        out[i] = std::upper_bound(seq.begin(), seq.end(), stops[i]) - seq.begin();
+    return out;
+}
+
+static int strand_waves(const StrandDev& d) { return (d.ncell + 63) / 64; }
+
+static Bundle empty_bundle() {
+    Bundle b;
+    memset(&b, 0, sizeof(b));
+    for (int w = 0; w < MAXS; ++w) b.wslot[w] = -1;
+    return b;
+}
+
+static void bundle_add(Bundle& b, int sidx, int wave0, int waves) {
+    const int slot = b.n++;
+    b.sidx[slot] = sidx;
+    b.wave0[slot] = wave0;
+    for (int w = wave0; w < wave0 + waves; ++w) b.wslot[w] = slot;
+}
+
+static int bundle_waves(const Bundle& b, const std::vector<StrandDev>& d) {
+    int u = 0;
+    for (int i = 0; i < b.n; ++i) u = std::max(u, b.wave0[i] + strand_waves(d[b.sidx[i]]));
+    return u;
+}
+
+static void set_kmax(std::vector<Bundle>& bs, const std::vector<StrandDev>& d) {
+    for (Bundle& b : bs) {
+        int64_t m = 0;
+        for (int i = 0; i < b.n; ++i) m = std::max(m, d[b.sidx[i]].nsteps);
+        b.kmax = m + 2;
+    }
+}
+
+// Layout 2: strands in decreasing step count are placed first-fit into
+// 8-wavefront bundles whose longest strand runs at most 1/0.7 times as many
+// steps (so that lanes of finished strands do not idle for long); bundles
+// are then ordered by their work, largest first.
+static std::vector<Bundle> pack_bundles(const std::vector<StrandDev>& d) {
+    const int n = (int)d.size();
+    std::vector<int> o(n);
+    std::iota(o.begin(), o.end(), 0);
+    std::stable_sort(o.begin(), o.end(), [&](int a, int b) {
+        if (d[a].nsteps != d[b].nsteps) return d[a].nsteps > d[b].nsteps;
+        return strand_waves(d[a]) > strand_waves(d[b]);
+    });
+    std::vector<Bundle> done, open;
+    std::vector<int> used;
+    std::vector<int64_t> top;
+    for (int i : o) {
+        const int w = strand_waves(d[i]);
+        // bundles whose longest strand is too long for this one (and every
+        // later one) are closed
+        for (size_t k = 0; k < open.size();) {
+            if ((double)d[i].nsteps < 0.7 * (double)top[k]) {
+                done.push_back(open[k]);
+                open.erase(open.begin() + k);
+                used.erase(used.begin() + k);
+                top.erase(top.begin() + k);
+            } else {
+                ++k;
+            }
+        }
+        size_t k = 0;
+        while (k < open.size() && !(used[k] + w <= MAXS && open[k].n < MAXS)) ++k;
+        if (k == open.size()) {
+            open.push_back(empty_bundle());
+            used.push_back(0);
+            top.push_back(d[i].nsteps);
+        }
+        bundle_add(open[k], i, used[k], w);
+        used[k] += w;
+    }
+    done.insert(done.end(), open.begin(), open.end());
+    set_kmax(done, d);
+    std::stable_sort(done.begin(), done.end(), [&](const Bundle& a, const Bundle& b) {
+        return (double)a.kmax * bundle_waves(a, d) > (double)b.kmax * bundle_waves(b, d);
+    });
+    return done;
 }
 
 extern "C" {
@@ -279,19 +382,24 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
         }
         s->phi_total = off;
         s->vrec_total = voff;
-        // Mode 1: every strand at the largest block, longest first (cost =
-        // step count x wavefronts).  Mode 0: strands grouped by their own
-        // wavefront count (one block-size class each), longest first inside
-        // a class.
-        auto waves = [&](int i) { return (desc[i].ncell + 63) / 64; };
-        auto cost = [&](int i) { return desc[i].stop_time * waves(i); };
-        std::vector<int> ord((size_t)2 * nstrands);
-        int* o0 = ord.data();
-        int* o1 = ord.data() + nstrands;
-        std::iota(o1, o1 + nstrands, 0);
-        std::stable_sort(o1, o1 + nstrands, [&](int a, int b) { return cost(a) > cost(b); });
-        std::copy(o1, o1 + nstrands, o0);
-        std::stable_sort(o0, o0 + nstrands, [&](int a, int b) { return waves(a) > waves(b); });
+        {
+            std::vector<double> stops(nstrands);
+            for (int i = 0; i < nstrands; ++i) stops[i] = desc[i].stop_time;
+            const std::vector<int64_t> ns = step_counts(stops, dt);
+            for (int i = 0; i < nstrands; ++i) s->hdev[i].nsteps = ns[i];
+        }
+        // Layout 1: every strand at the largest block, most work first (step
+        // count x wavefronts).  Layout 0: strands grouped by their own
+        // wavefront count (one block-size class each), most work first inside
+        // a class.  Layout 2: packed bundles.
+        const std::vector<StrandDev>& H = s->hdev;
+        auto waves = [&](int i) { return strand_waves(H[i]); };
+        auto cost = [&](int i) { return (double)H[i].nsteps * waves(i); };
+        std::vector<int> o1(nstrands);
+        std::iota(o1.begin(), o1.end(), 0);
+        std::stable_sort(o1.begin(), o1.end(), [&](int a, int b) { return cost(a) > cost(b); });
+        std::vector<int> o0 = o1;
+        std::stable_sort(o0.begin(), o0.end(), [&](int a, int b) { return waves(a) > waves(b); });
         int dev = 0;
         LQT_HIP(hipGetDevice(&dev));
         hipDeviceProp_t prop;
@@ -304,11 +412,27 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
             int j = i;
             while (j < nstrands && waves(o0[j]) == waves(o0[i])) ++j;
             const int b = 64 * waves(o0[i]);
-            s->layout[0].push_back({b, grid_of(b, j - i), i, j - i});
+            s->layout[0].push_back({b, grid_of(b, j - i), (int)s->hbund.size(), j - i});
+            for (int k = i; k < j; ++k) {
+                Bundle bd = empty_bundle();
+                bundle_add(bd, o0[k], 0, waves(o0[k]));
+                s->hbund.push_back(bd);
+            }
             i = j;
         }
         const int bmax = std::min(512, (maxl + 63) / 64 * 64);
-        s->layout[1].push_back({bmax, grid_of(bmax, nstrands), 0, nstrands});
+        s->layout[1].push_back({bmax, grid_of(bmax, nstrands), (int)s->hbund.size(), nstrands});
+        for (int k = 0; k < nstrands; ++k) {
+            Bundle bd = empty_bundle();
+            bundle_add(bd, o1[k], 0, waves(o1[k]));
+            s->hbund.push_back(bd);
+        }
+        {
+            std::vector<Bundle> pk = pack_bundles(H);
+            s->layout[2].push_back({512, grid_of(512, (int)pk.size()), (int)s->hbund.size(), (int)pk.size()});
+            s->hbund.insert(s->hbund.end(), pk.begin(), pk.end());
+        }
+        set_kmax(s->hbund, H);
         s->block = bmax;
         std::vector<uint8_t> cl(ncells);
         for (int64_t i = 0; i < ncells; ++i) cl[i] = (uint8_t)cell_class(celltype[i]);
@@ -318,7 +442,7 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
         for (int i = 0; i < nparams; ++i) v0[i] = params[i].v0;
         lqt_status e;
         if ((e = upload(&s->ddesc, s->hdev.data(), nstrands))) return e;
-        if ((e = upload(&s->order, ord.data(), ord.size()))) return e;  // both modes' orders
+        if ((e = upload(&s->bund, s->hbund.data(), s->hbund.size()))) return e;  // all layouts' bundles
         if ((e = upload(&s->queue, (const int*)nullptr, 8))) return e;
         if ((e = upload(&s->cls, cl.data(), ncells))) return e;
         if ((e = upload(&s->coeff, co.data(), ncells))) return e;
@@ -359,7 +483,7 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
 lqt_status lqt_strands_destroy(lqt_strands* s) {
     if (!s) return LQT_OK;
     if (s->stream) hipStreamSynchronize(s->stream);
-    for (int m = 0; m < 2; ++m) {
+    for (int m = 0; m < NMODES; ++m) {
         if (s->exec_m[m]) hipGraphExecDestroy(s->exec_m[m]);
         if (s->graph_m[m]) hipGraphDestroy(s->graph_m[m]);
     }
@@ -369,7 +493,7 @@ lqt_status lqt_strands_destroy(lqt_strands* s) {
     }
     if (s->fork) hipEventDestroy(s->fork);
     hipFree(s->ddesc);
-    hipFree(s->order);
+    hipFree(s->bund);
     hipFree(s->queue);
     hipFree(s->cls);
     hipFree(s->coeff);
@@ -559,34 +683,50 @@ lqt_status lqt_strands_block(const lqt_strands* s, int* block) {
 
 lqt_status lqt_strands_autotune(lqt_strands* s, int trial_steps, int* mode, double* ms) {
     LQT_CHECK(s && trial_steps > 0, LQT_ERR_ARG, "lqt_strands_autotune: bad argument");
-    // the same batch cut to trial_steps steps, timed in both layouts
+    // The batch with every stop time scaled by the same factor, so that the
+    // longest strand runs trial_steps steps and the strands keep their
+    // relative lengths (the order in which they finish decides the layout),
+    // timed in every layout.
+    double smax = 0.0;
+    for (const auto& d : s->hdev) smax = std::max(smax, d.stop_time);
+    const double f = std::min(1.0, trial_steps * s->dt / std::max(smax, s->dt));
     std::vector<StrandDev> cut = s->hdev;
-    for (auto& d : cut) d.stop_time = std::min(d.stop_time, (trial_steps - 0.5) * s->dt);
+    std::vector<double> stops(s->n);
+    for (int i = 0; i < s->n; ++i) stops[i] = cut[i].stop_time * f;
+    const std::vector<int64_t> ns = step_counts(stops, s->dt);
+    for (int i = 0; i < s->n; ++i) {
+        cut[i].stop_time = stops[i];
+        cut[i].nsteps = ns[i];
+    }
+    std::vector<Bundle> cb = s->hbund;
+    set_kmax(cb, cut);
     LQT_HIP(hipMemcpy(s->ddesc, cut.data(), sizeof(StrandDev) * s->n, hipMemcpyHostToDevice));
-    double t[2] = {0, 0};
+    LQT_HIP(hipMemcpy(s->bund, cb.data(), sizeof(Bundle) * cb.size(), hipMemcpyHostToDevice));
+    double t[NMODES] = {0, 0, 0};
     lqt_status e = LQT_OK;
-    for (int m = 0; m < 2 && !e; ++m) {
+    for (int m = 0; m < NMODES && !e; ++m) {
         double w = 0;
         e = run_mode(s, m, &w);  // first replay: warm
         if (!e) e = run_mode(s, m, &t[m]);
     }
     LQT_HIP(hipMemcpy(s->ddesc, s->hdev.data(), sizeof(StrandDev) * s->n, hipMemcpyHostToDevice));
+    LQT_HIP(hipMemcpy(s->bund, s->hbund.data(), sizeof(Bundle) * s->hbund.size(), hipMemcpyHostToDevice));
     if (e) return e;
-    s->mode = t[0] <= t[1] ? 0 : 1;
-    s->tune_ms[0] = t[0];
-    s->tune_ms[1] = t[1];
+    int best = 0;
+    for (int m = 1; m < NMODES; ++m)
+        if (t[m] < t[best]) best = m;
+    s->mode = best;
+    for (int m = 0; m < NMODES; ++m) s->tune_ms[m] = t[m];
     s->tuned = true;
     s->ran = false;
     if (mode) *mode = s->mode;
-    if (ms) {
-        ms[0] = t[0];
-        ms[1] = t[1];
-    }
+    if (ms)
+        for (int m = 0; m < NMODES; ++m) ms[m] = t[m];
     return LQT_OK;
 }
 
 lqt_status lqt_strands_set_mode(lqt_strands* s, int mode) {
-    LQT_CHECK(s && (mode == 0 || mode == 1), LQT_ERR_ARG, "lqt_strands_set_mode: mode must be 0 or 1");
+    LQT_CHECK(s && mode >= 0 && mode < NMODES, LQT_ERR_ARG, "lqt_strands_set_mode: mode must be 0, 1 or 2");
     s->mode = mode;
     return LQT_OK;
 }

@@ -1,17 +1,16 @@
-// Persistent batched-strand kernel.  One workgroup owns one strand at a
-// time (cells on lanes, state in registers for the whole run), pulls the
-// next strand from a work counter when done, and advances its strand with
-// the loop of OneD.cc:315-436: Jacobi snapshot of V in LDS, no-flux ends
-// by copy, S1 on the first cells by the in-cycle time of the double time
-// accumulator (or, while that window is closed, the S2 of
+// Persistent batched-strand kernel.  A workgroup takes one bundle of
+// strands at a time from a work counter (each strand on whole wavefronts,
+// cells on lanes, state in registers for the whole run) and advances every
+// strand of the bundle with the loop of OneD.cc:315-436: Jacobi snapshot of
+// V in LDS, no-flux ends by copy, S1 on the first cells by the in-cycle time
+// of the double time accumulator (or, while that window is closed, the S2 of
 // OneD_VW.cc:291-301), then the TPORd update.  The pseudo-ECG of
-// PlotScript/plot1DECG_compare.py:89-105 is reduced across the workgroup
-// every rec_every steps (wavefront shuffles, then one LDS slot per wave).
+// PlotScript/plot1DECG_compare.py:89-105 is reduced across each strand every
+// rec_every steps (wavefront shuffles, then one LDS slot per wavefront,
+// summed by the strand's first lane one step later).
 #include "strands.hpp"
 
 namespace lqt {
-
-constexpr int MAXB = 512;
 
 template <class R>
 __device__ __forceinline__ R wave_sum(R x) {
@@ -26,8 +25,10 @@ __device__ __forceinline__ R wave_sum(R x) {
 template <class R, int B>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 2 : 1))) void strands_kernel(
     StrandsArgs<R> A) {
-    __shared__ R vbuf[2][B + 2];
-    __shared__ float part[2][B / 64];
+    constexpr int NW = B / 64;
+    // one V segment per strand: its cells plus a mirrored end on each side
+    __shared__ R vbuf[2][B + 2 * MAXS];
+    __shared__ float part[2][NW];
     __shared__ int s_next;
     const int tid = threadIdx.x;
     const int wid = tid >> 6;
@@ -37,12 +38,22 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
         __syncthreads();
         const int q = s_next;
         __syncthreads();
-        if (q >= A.nstrands) break;
-        const int sidx = A.order[q];
+        if (q >= A.nbundles) break;
+        const Bundle& Q = A.bundles[q];
+        const int64_t kmax = Q.kmax;
+        const int slot = Q.wslot[wid];
+        const bool has = slot >= 0;
+        const int sidx = has ? Q.sidx[slot] : 0;
+        const int w0 = has ? Q.wave0[slot] : 0;
         const StrandDev D = A.desc[sidx];
-        const int n = D.ncell;
-        const bool live = tid < n;
-        const int cell = D.cell_off + (live ? tid : 0);
+        const int n = has ? D.ncell : 0;
+        const int j = tid - 64 * w0;        // cell of this lane in its strand
+        const bool live = has && j < n;
+        const bool lead = has && j == 0;    // writes the strand's own outputs
+        const int seg = 64 * w0 + 2 * slot; // LDS segment of the strand
+        const int p = seg + 1 + j;          // this cell's LDS slot
+        const int nws = (n + 63) >> 6;      // wavefronts of the strand
+        const int cell = D.cell_off + (live ? j : 0);
 
         // diffusion coefficient and its derivative (OneD.cc:108-131)
         const double dxd = D.dx;
@@ -50,8 +61,8 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
         if (live) {
             Di = A.coeff[cell];
             if (n == 1) dDi = 0.0;
-            else if (tid == 0) dDi = (A.coeff[cell + 1] - A.coeff[cell]) / dxd;
-            else if (tid == n - 1) dDi = (A.coeff[cell] - A.coeff[cell - 1]) / dxd;
+            else if (j == 0) dDi = (A.coeff[cell + 1] - A.coeff[cell]) / dxd;
+            else if (j == n - 1) dDi = (A.coeff[cell] - A.coeff[cell - 1]) / dxd;
             else dDi = (A.coeff[cell + 1] - A.coeff[cell - 1]) / (2.0 * dxd);
         }
         const R RD = (R)Di, RdD = (R)dDi;
@@ -59,8 +70,8 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
         // pseudo-ECG weight of this cell: one-sided at the ends
         R wphi = R(0);
         if (live) {
-            const double r = (tid * dxd - D.x0) * (tid * dxd - D.x0);
-            wphi = (R)((tid == 0 || tid == n - 1) ? 1.0 / r : 1.0 / (2.0 * r));
+            const double r = (j * dxd - D.x0) * (j * dxd - D.x0);
+            wphi = (R)((j == 0 || j == n - 1) ? 1.0 / r : 1.0 / (2.0 * r));
         }
         const int c = live ? A.cls[cell] : 0;
         const Consts<R> cc = A.consts[D.pidx];
@@ -80,105 +91,96 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
         // peak V after the S2 start (OneD_VW.cc:206-207,390-394); only
         // cells 0 and n-1 report it
         double wmax = -100.0;
-        const bool s2_cell = tid >= D.s2_first && tid < D.s2_first + D.s2_count;
+        const bool s2_cell = j >= D.s2_first && j < D.s2_first + D.s2_count;
 
         int64_t nrec = 0, nvrow = 0;
-        int pend = 0;         // a sample waits for its reduction
-        int fin_slot = -1;     // slot whose per-wave partials wait for thread 0
+        int pend = 0;          // a sample waits for its reduction
+        int fin_slot = -1;     // slot whose per-wave partials wait for the lead lane
         int64_t fin_idx = 0;
         int64_t pend_idx = 0;
         double time = 0.0;
-        int64_t step = 0;
         int b = 0;
-        for (;; time += A.dt_d, ++step) {
-            const bool run = time <= D.stop_time;
+        for (int64_t k = 0; k < kmax; ++k, time += A.dt_d) {
+            const bool run = has && k < D.nsteps;
             // V^n into LDS with mirrored ends
             if (live) {
-                vbuf[b][tid + 1] = v;
-                if (tid == 0) vbuf[b][0] = v;
-                if (tid == n - 1) vbuf[b][n + 1] = v;
+                vbuf[b][p] = v;
+                if (j == 0) vbuf[b][seg] = v;
+                if (j == n - 1) vbuf[b][seg + n + 1] = v;
             }
             __syncthreads();
             // finish the sample whose per-wave partials were written last step
-            if (fin_slot >= 0 && tid == 0) {
+            if (fin_slot >= 0 && lead) {
                 float tot = 0.f;
-                for (int w = 0; w < B / 64; ++w) tot += part[fin_slot][w];
+                for (int w = 0; w < nws; ++w) tot += part[fin_slot][w0 + w];
                 A.phi[D.phi_off + fin_idx] = tot;
             }
             fin_slot = -1;
             // reduce the sample recorded at the end of the previous step
             if (pend) {
-                const R dv = live ? vbuf[b][tid + 2] - vbuf[b][tid] : R(0);
+                const R dv = live ? vbuf[b][p + 1] - vbuf[b][p - 1] : R(0);
                 const R tot = wave_sum<R>(-wphi * dv);
-                const int slot = (int)(pend_idx & 1);
-                if ((tid & 63) == 0) part[slot][wid] = (float)tot;
-                fin_slot = slot;
+                const int sl = (int)(pend_idx & 1);
+                if ((tid & 63) == 0) part[sl][wid] = (float)tot;
+                fin_slot = sl;
                 fin_idx = pend_idx;
                 pend = 0;
             }
-            if (!run) {
-                // make the last partials visible, then finish them
-                __syncthreads();
-                if (fin_slot >= 0 && tid == 0) {
-                    float tot = 0.f;
-                    for (int w = 0; w < B / 64; ++w) tot += part[fin_slot][w];
-                    A.phi[D.phi_off + fin_idx] = tot;
-                }
-                break;
-            }
-            if (live) {
-                const R vl = vbuf[b][tid], vr = vbuf[b][tid + 2], vc = vbuf[b][tid + 1];
-                const R first = (tid == 0 || tid == n - 1) ? (vr - vl) * inv_dx : (vr - vl) * inv_2dx;
-                const R second = (vr + vl - R(2) * vc) * inv_dx2;
-                const R dvgap = RdD * first + RD * second;
-                const double tc = time - floor(time / D.bcl) * D.bcl;
-                const double t2 = time - D.s2_base;
-                R ist = R(0);
-                if (tc >= D.stim_start && tc < D.stim_start + D.stim_dur) {
-                    if (tid < D.nstim) ist = (R)D.stim_strength;
-                } else if (t2 >= D.s2_start && t2 < D.s2_start + D.s2_dur && s2_cell) {
-                    ist = (R)D.s2_strength;
-                }
-                v = tpord_update<R, false>(s, v, c, cc, A.dt, ist, dvgap, nullptr);
-                if (t2 >= D.s2_start && (double)v >= wmax) wmax = (double)v;
+            if (run) {
+                if (live) {
+                    const R vl = vbuf[b][p - 1], vr = vbuf[b][p + 1], vc = vbuf[b][p];
+                    const R first = (j == 0 || j == n - 1) ? (vr - vl) * inv_dx : (vr - vl) * inv_2dx;
+                    const R second = (vr + vl - R(2) * vc) * inv_dx2;
+                    const R dvgap = RdD * first + RD * second;
+                    const double tc = time - floor(time / D.bcl) * D.bcl;
+                    const double t2 = time - D.s2_base;
+                    R ist = R(0);
+                    if (tc >= D.stim_start && tc < D.stim_start + D.stim_dur) {
+                        if (j < D.nstim) ist = (R)D.stim_strength;
+                    } else if (t2 >= D.s2_start && t2 < D.s2_start + D.s2_dur && s2_cell) {
+                        ist = (R)D.s2_strength;
+                    }
+                    v = tpord_update<R, false>(s, v, c, cc, A.dt, ist, dvgap, nullptr);
+                    if (t2 >= D.s2_start && (double)v >= wmax) wmax = (double)v;
 
-                // beat tracker, in the measured beat only (floor(t/BCL) == k)
-                if (floor(time / D.bcl) == D.measure_beat) {
-                    const double vf = (double)v;
-                    if (!(tr.flags & 1)) {
-                        tr.flags |= 1;
-                        tr.vrest = vf;
-                    }
-                    if (!(tr.flags & 2)) {
-                        if (vf >= -30.0) {
-                            tr.flags |= 2;
-                            tr.act = time;
-                            tr.vmax = vf;
+                    // beat tracker, in the measured beat only (floor(t/BCL) == k)
+                    if (floor(time / D.bcl) == D.measure_beat) {
+                        const double vf = (double)v;
+                        if (!(tr.flags & 1)) {
+                            tr.flags |= 1;
+                            tr.vrest = vf;
                         }
-                    } else if (!(tr.flags & 4)) {
-                        if (vf > tr.vmax) tr.vmax = vf;
-                        const double lvl = tr.vmax - 0.9 * (tr.vmax - tr.vrest);
-                        if (vf <= lvl) {
-                            tr.flags |= 4;
-                            tr.rep = time;
-                        } else if (time >= tr.act + 100.0) {
-                            // phase 2/3: a rise of more than 2 mV above the
-                            // lowest V since act+100 ms is an EAD
-                            if (vf < tr.vmin) tr.vmin = vf;
-                            if (vf > tr.vmin + 2.0) tr.flags |= 8;
+                        if (!(tr.flags & 2)) {
+                            if (vf >= -30.0) {
+                                tr.flags |= 2;
+                                tr.act = time;
+                                tr.vmax = vf;
+                            }
+                        } else if (!(tr.flags & 4)) {
+                            if (vf > tr.vmax) tr.vmax = vf;
+                            const double lvl = tr.vmax - 0.9 * (tr.vmax - tr.vrest);
+                            if (vf <= lvl) {
+                                tr.flags |= 4;
+                                tr.rep = time;
+                            } else if (time >= tr.act + 100.0) {
+                                // phase 2/3: a rise of more than 2 mV above the
+                                // lowest V since act+100 ms is an EAD
+                                if (vf < tr.vmin) tr.vmin = vf;
+                                if (vf > tr.vmin + 2.0) tr.flags |= 8;
+                            }
                         }
                     }
                 }
-            }
-            if (D.rec_every > 0 && (step % D.rec_every) == 0 && time >= D.rec_from && nrec < D.phi_cap) {
-                if (nrec == 0 && tid == 0) A.phit0[sidx] = time;
-                pend = 1;
-                pend_idx = nrec++;
-            }
-            if (D.vrec_every > 0 && (step % D.vrec_every) == 0 && time >= D.rec_from && nvrow < D.vrec_cap) {
-                if (live) A.vrec[D.vrec_off + nvrow * (int64_t)n + tid] = (double)v;
-                if (nvrow == 0 && tid == 0) A.vrect0[sidx] = time;
-                ++nvrow;
+                if (D.rec_every > 0 && (k % D.rec_every) == 0 && time >= D.rec_from && nrec < D.phi_cap) {
+                    if (nrec == 0 && lead) A.phit0[sidx] = time;
+                    pend = 1;
+                    pend_idx = nrec++;
+                }
+                if (D.vrec_every > 0 && (k % D.vrec_every) == 0 && time >= D.rec_from && nvrow < D.vrec_cap) {
+                    if (live) A.vrec[D.vrec_off + nvrow * (int64_t)n + j] = (double)v;
+                    if (nvrow == 0 && lead) A.vrect0[sidx] = time;
+                    ++nvrow;
+                }
             }
             b ^= 1;
         }
@@ -192,13 +194,13 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
             A.ead[cell] = e;
             store_state(s, A.fin, cell);
             A.finv[cell] = v;
-            if (tid == 0) A.vmax[2 * (int64_t)sidx] = wmax;
-            if (tid == n - 1) A.vmax[2 * (int64_t)sidx + 1] = wmax;
+            if (j == 0) A.vmax[2 * (int64_t)sidx] = wmax;
+            if (j == n - 1) A.vmax[2 * (int64_t)sidx + 1] = wmax;
         }
-        if (tid == 0) {
+        if (lead) {
             A.nvrec[sidx] = nvrow;
             A.nphi[sidx] = nrec;
-            A.steps[sidx] = step;
+            A.steps[sidx] = D.nsteps;
         }
         __syncthreads();
     }

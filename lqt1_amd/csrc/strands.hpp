@@ -14,6 +14,24 @@ struct StrandDev {
     // S2 (OneD_VW.cc:291-301): only while the S1 window is closed
     double s2_base, s2_start, s2_dur, s2_strength;
     int s2_first, s2_count;
+    // steps of `for (time = 0; time <= stop_time; time += dt)`, counted on
+    // the host with the same double accumulator
+    int64_t nsteps;
+};
+
+// Strands that share one workgroup: strand s occupies the whole wavefronts
+// wave0[s] .. wave0[s] + ceil(ncell/64) - 1, so every wavefront belongs to
+// one strand and its shuffles and partial sums stay inside that strand.
+// All strands of a bundle advance in lock-step on one time accumulator;
+// the loop runs kmax = max(nsteps) + 2 iterations (one to reduce the last
+// ECG sample, one to write it out).
+constexpr int MAXS = 8;
+struct Bundle {
+    int n;
+    int sidx[MAXS];
+    int wave0[MAXS];
+    int wslot[MAXS];  // strand slot of wavefront w, -1 if unused
+    int64_t kmax;
 };
 
 // Per-cell beat tracker: -30 mV activation (OneD.cc:422), running peak,
@@ -26,10 +44,10 @@ struct CellTrack {
 
 template <class R>
 struct StrandsArgs {
-    int nstrands;
-    const StrandDev* desc;
-    const int* order;          // strand processing order (longest first)
+    int nbundles;
+    const Bundle* bundles;     // this launch's bundles, processing order
     int* queue;                // work counter, zeroed before each run
+    const StrandDev* desc;
     const uint8_t* cls;        // per cell class
     const double* coeff;       // per cell D (mm^2/ms)
     const Consts<R>* consts;

@@ -91,10 +91,12 @@ class Strands:
         return v.value
 
     def autotune(self, trial_steps=2000):
-        """Time both launch layouts on the batch cut to trial_steps steps and
-        keep the faster; returns (mode, [ms mode 0, ms mode 1])."""
+        """Time every launch layout (0: per-size classes, 1: one strand per
+        workgroup at the largest block, 2: packed bundles) on the batch with
+        its stop times scaled so that the longest strand runs trial_steps
+        steps, and keep the fastest; returns (mode, [ms per layout])."""
         mode = C.c_int()
-        ms = np.zeros(2)
+        ms = np.zeros(3)
         L.check(L.lib().lqt_strands_autotune(self.h, int(trial_steps), C.byref(mode), L.ptr(ms)))
         return mode.value, ms.tolist()
 

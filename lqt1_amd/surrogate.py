@@ -133,13 +133,14 @@ def spec_of(p: Point, nbeats: int) -> StrandSpec:
 
 
 def simulate(points, nbeats=10, precision=L.FP32, chunk=4096, progress=None):
-    """Run every point as a strand through the batched solver; returns the
-    per-strand results and the device time.  progress(done, total) is
-    called after each chunk."""
+    """Run every point as a strand through the batched solver (launch layout
+    chosen by the autotuner per chunk); returns the per-strand results and
+    the device time.  progress(done, total) is called after each chunk."""
     out, t = [], 0.0
     for i in range(0, len(points), chunk):
         st = Strands([spec_of(p, nbeats) for p in points[i:i + chunk]], precision, 0.02)
         t0 = time.perf_counter()
+        st.autotune(2000)
         st.run()
         t += time.perf_counter() - t0
         out.extend(st.results())

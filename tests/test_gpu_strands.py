@@ -45,9 +45,13 @@ def oracle_runs(oracle, lqt):
     return [_oracle(oracle, s) for s in _specs(lqt)]
 
 
-def test_strands_fp64_match_oracle(lqt, oracle_runs):
+@pytest.mark.parametrize("mode", [0, 2])
+def test_strands_fp64_match_oracle(lqt, oracle_runs, mode):
+    """Layout 2 puts the 57-, 100- and 80-cell strands (60001, 50001 and
+    50001 steps) in one workgroup and the 130-cell strand in another."""
     specs = _specs(lqt)
     st = lqt.Strands(specs, lqt.FP64, 0.02)
+    st.set_mode(mode)
     st.run()
     res = st.results()
     for i, (s, r) in enumerate(zip(specs, oracle_runs)):
@@ -89,13 +93,14 @@ def test_strands_fp32_within_gates(lqt, oracle_runs):
 
 
 def test_launch_layouts_agree_and_autotune(lqt):
-    """Both launch layouts (per-size classes in parallel branches, or one
-    kernel at the largest block) give identical per-strand results; each
-    layout runs on its own handle so that a strand a layout skipped would
-    show up as missing output, and every strand reports its full step count."""
+    """Every launch layout (per-size classes in parallel branches, one kernel
+    at the largest block, strands packed several to a workgroup) gives
+    identical per-strand results; each layout runs on its own handle so that
+    a strand a layout skipped would show up as missing output, and every
+    strand reports its full step count."""
     specs = _specs(lqt)
     outs = []
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         h = lqt.Strands(specs, lqt.FP32, 0.02)
         h.set_mode(mode)
         h.run()
@@ -109,23 +114,26 @@ def test_launch_layouts_agree_and_autotune(lqt):
         outs.append([h.cell_results(i) for i in range(len(specs))] + [h.ecg(i)[1] for i in range(len(specs))])
         h.close()
     st = lqt.Strands(specs, lqt.FP32, 0.02)
-    for k, (a, b) in enumerate(zip(outs[0], outs[1])):
-        if isinstance(a, tuple):
-            for x, y in zip(a, b):
-                bad = np.flatnonzero(x != y)
-                assert bad.size == 0, (k, bad, x[bad], y[bad])
-        else:
-            assert np.array_equal(a, b)
+    for other in outs[1:]:
+        for k, (a, b) in enumerate(zip(outs[0], other)):
+            if isinstance(a, tuple):
+                for x, y in zip(a, b):
+                    bad = np.flatnonzero(x != y)
+                    assert bad.size == 0, (k, bad, x[bad], y[bad])
+            else:
+                assert np.array_equal(a, b)
     mode, ms = st.autotune(500)
-    assert mode in (0, 1) and ms[mode] <= ms[1 - mode]
+    assert mode in (0, 1, 2) and ms[mode] == min(ms)
     st.run()
     act, _, _ = st.cell_results(0)
     assert np.array_equal(act, outs[0][0][0])
 
 
-def test_graph_replays_are_identical(lqt):
+@pytest.mark.parametrize("mode", [0, 2])
+def test_graph_replays_are_identical(lqt, mode):
     specs = _specs(lqt)[1:3]
     st = lqt.Strands(specs, lqt.FP32, 0.02)
+    st.set_mode(mode)
     outs = []
     for _ in range(3):
         st.run()
