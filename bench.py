@@ -117,10 +117,9 @@ def strand_accuracy(A):
                 d_qt_ms=float(abs(q32 - q64)))
 
 
-def strands_throughput(A, nstr, nbeats, seed=0):
-    """Batched strand sweep: nstr strands, length U{50..400}, IKs U[0,1],
-    ISO {0,1}, dose U[0,1]*300 fmol, BCL {600,800,1000,1500}, M fraction
-    U[0.2,0.5], nbeats beats, one persistent launch."""
+def strand_specs(A, nstr, nbeats, seed=0):
+    """The sweep's strands: length U{50..400}, IKs U[0,1], ISO {0,1}, dose
+    U[0,1]*300 fmol, BCL {600,800,1000,1500}, M fraction U[0.2,0.5]."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -132,6 +131,43 @@ def strands_throughput(A, nstr, nbeats, seed=0):
         mf = float(rng.uniform(0.2, 0.5))
         specs.append(A.StrandSpec(A.transmural(L, 0.5 * (1 - mf), mf), p,
                                   bcl=float(rng.choice([600, 800, 1000, 1500])), nbeats=nbeats))
+    return specs
+
+
+def strands_cpu(specs, nsample, threads):
+    """The FP64 oracle's strand loop (OneD.cc) on the first nsample strands
+    of the sweep for one beat each, one strand per host thread; returns
+    strands/s scaled to the sweep's beats and the sample's cell-steps/s."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle as O
+
+    O.set_num_threads(1)
+
+    def one(sp):
+        p = sp.params
+        op = O.params(iks_scale=p.iks_scale, iso=p.iso, pca_iso=p.pca_iso, suprep=p.suprep, dose_fmol=p.dose_fmol)
+        r = O.strand_run(sp.celltype, sp.coeff, op, dx=sp.dx, bcl=sp.bcl, stop_time=sp.bcl, stim=sp.stim_strength,
+                         stim_dur=sp.stim_duration, stim_start=sp.stim_start, nstim=sp.nstim, record=False)
+        return r["steps"] * sp.celltype.size
+
+    sample = specs[:nsample]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        cs = sum(ex.map(one, sample))
+    wall = time.perf_counter() - t0
+    beats = specs[0].nbeats
+    return dict(value=nsample / beats / wall, unit="strands/s", cores=threads, kind="port",
+                cell_steps_per_s=cs / wall,
+                sample=f"FP64 oracle strand loop, first {nsample} strands of the sweep for 1 of their {beats} "
+                       f"beats, one strand per thread, {wall:.1f} s")
+
+
+def strands_throughput(A, nstr, nbeats, seed=0):
+    """Batched strand sweep (strand_specs), one persistent launch."""
+    import numpy as np
+
+    specs = strand_specs(A, nstr, nbeats, seed)
     st = A.Strands(specs, A.FP32, 0.02)
     mode, trial = st.autotune(2000)  # also captures both graphs
     t0 = time.perf_counter()
@@ -143,6 +179,7 @@ def strands_throughput(A, nstr, nbeats, seed=0):
                layout_mode=mode, trial_ms=trial, max_block=st.block(), ead=int(sum(x["ead"] for x in r)),
                qt_median_ms=float(np.median([x["qt"] for x in r if x["qt"] > 0])))
     st.close()
+    out["_specs"] = specs
     return out
 
 
@@ -246,6 +283,7 @@ def main():
     if not args.no_strands:
         try:
             strands = strands_throughput(A, 512, args.strand_beats)
+            specs = strands.pop("_specs")
             log(f"strands: {strands}")
         except Exception as e:
             strands = dict(error=str(e))
@@ -259,6 +297,9 @@ def main():
             cpu = dict(value=cps, unit="cell-steps/s", cores=threads, kind="port",
                        sample=f"FP64 oracle tissue step (diffusion + TPORd update, OpenMP over voxels), "
                               f"{args.cpu_steps} steps of the same {nvox}-voxel ventricle, {wall:.1f} s")
+            if strands is not None and "error" not in strands:
+                strands["cpu_baseline"] = strands_cpu(specs, threads, threads)
+                strands["gpu_over_cpu"] = strands["strands_per_s"] / strands["cpu_baseline"]["value"]
         except Exception as e:
             cpu = dict(error=str(e))
 
