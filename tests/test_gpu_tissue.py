@@ -150,7 +150,7 @@ def test_launch_shapes_agree_and_autotune(lqt, prec):
     P = lqt.FP32 if prec == "fp32" else lqt.FP64
     lab, fib, sites = _blob(3, (16, 14, 12))
     shapes = lqt.Tissue.variants()
-    assert len(shapes) >= 4 and shapes[0] == (256, True, 2)
+    assert len(shapes) >= 4 and shapes[0] == (256, False, 2)
     ref = None
     for i in range(len(shapes)):
         T = lqt.Tissue(lab, fib, 0.2, 0.154, 0.0385, lqt.preset("cpu_default"), P, 0.02)
