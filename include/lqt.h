@@ -189,10 +189,13 @@ lqt_status lqt_tissue_get_variant(const lqt_tissue *t, int *i);
  * us[count] (may be NULL) receives each shape's mean kernel time. */
 lqt_status lqt_tissue_autotune(lqt_tissue *t, int trial_steps, int *best, double *us);
 /* Average device time of the fused step kernel over the last call of
- * lqt_tissue_step, in microseconds (HIP events on the tissue's stream). */
+ * lqt_tissue_step, in microseconds (HIP events on the tissue's stream, on
+ * every 16th step).  The step kernel refreshes the ghost cells itself, so
+ * *us_ghost_kernel is 0. */
 lqt_status lqt_tissue_last_kernel_us(const lqt_tissue *t, double *us_step_kernel, double *us_ghost_kernel);
 /* Algorithmic HBM bytes of one fused step kernel launch (per voxel state,
- * V and static data, read and written once). */
+ * V and static data, read and written once; per ghost cell its index read
+ * and V written). */
 lqt_status lqt_tissue_bytes_per_step(const lqt_tissue *t, double *bytes);
 
 /* ---------------------------------------------------------------------------

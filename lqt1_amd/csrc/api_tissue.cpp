@@ -34,8 +34,10 @@ struct lqt_tissue {
     double* d = nullptr;
     void* fib = nullptr;
     void* drift = nullptr;
-    int32_t* gdst = nullptr;
-    int32_t* gsrc = nullptr;
+    int32_t* gdst = nullptr;   // ghost cells (padded index) and their sources,
+    int32_t* gsrc = nullptr;   // for refreshes outside the step kernel
+    int32_t* gout = nullptr;   // ghost cells listed by the voxel they mirror
+    int32_t* gwave = nullptr;  // first gout entry of each wavefront's voxels
     void* consts = nullptr;
     float* stim_off = nullptr;
     uint8_t* s2 = nullptr;
@@ -188,16 +190,37 @@ static void ghost_map(int nx, int ny, int nz, const uint8_t* label, std::vector<
     if (nghost) *nghost = ng;
 }
 
-// Padded destination / source grid indices of every ghost cell.
+// Padded destination / source grid indices of every ghost cell, and the
+// tissue voxel each one mirrors.
 static void build_ghosts(const lqt_tissue* t, const std::vector<int32_t>& g, std::vector<int32_t>& dst,
-                         std::vector<int32_t>& src) {
+                         std::vector<int32_t>& src, std::vector<int32_t>& srck) {
     const int64_t ncell = (int64_t)t->nx * t->ny * t->nz;
     for (int64_t c = 0; c < ncell; ++c) {
         if (g[c] >= 0) continue;
         const int x = (int)(c / ((int64_t)t->ny * t->nz)), y = (int)((c / t->nz) % t->ny), z = (int)(c % t->nz);
         dst.push_back((int32_t)(((int64_t)(x + 1) * t->py + (y + 1)) * t->pz + (z + 1)));
         src.push_back(t->h_vpos[-g[c] - 1]);
+        srck.push_back(-g[c] - 1);
     }
+}
+
+// The step kernel refreshes the ghosts itself: voxel k writes its new V to
+// the ghost cells that mirror it.  Their destinations are listed by voxel
+// (gout); voxel k finds its first one at gwave[k/64] plus the ghosts of the
+// lower lanes of its wavefront (a prefix sum over the counts that bits 2-6
+// of cls[k] carry, at most 26 for the 26 neighbours).
+static void ghosts_by_voxel(int64_t nvox, const std::vector<int32_t>& dst, const std::vector<int32_t>& srck,
+                            std::vector<uint8_t>& cl, std::vector<int32_t>& gout, std::vector<int32_t>& gwave) {
+    std::vector<int32_t> cnt(nvox + 1, 0);
+    for (int32_t k : srck) cnt[k + 1] += 1;
+    for (int64_t k = 0; k < nvox; ++k) cl[k] = (uint8_t)(cl[k] | (cnt[k + 1] << 2));
+    for (int64_t k = 0; k < nvox; ++k) cnt[k + 1] += cnt[k];  // cnt[k] = first ghost of voxel k
+    gout.assign(dst.size(), 0);
+    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+    for (size_t i = 0; i < dst.size(); ++i) gout[fill[srck[i]]++] = dst[i];
+    const int64_t nw = (nvox + 63) / 64;
+    gwave.resize(nw);
+    for (int64_t w = 0; w < nw; ++w) gwave[w] = cnt[64 * w];
 }
 
 template <class R>
@@ -229,6 +252,8 @@ static void fill_args(lqt_tissue* t, TissueArgs<R>& a) {
     a.s2_strength = (R)t->proto.s2_strength;
     a.s2 = t->s2;
     a.guard = t->guard;
+    a.gout = t->gout;
+    a.gwave = t->gwave;
 }
 
 static lqt_status tissue_upload_state(lqt_tissue* t, const std::vector<double>& s40);
@@ -286,12 +311,13 @@ lqt_status lqt_tissue_create(const lqt_geometry* geo, const lqt_params* params, 
         t->h_label[k] = geo->label[c];
         cl[k] = (uint8_t)(cell_class(geo->label[c] & 0x7f) | (geo->label[c] & LQT_HOLD));
     }
-    std::vector<int32_t> gsigned, gd, gs;
+    std::vector<int32_t> gsigned, gd, gs, gk, gout, gwave;
     ghost_map(t->nx, t->ny, t->nz, geo->label, gsigned, nullptr, nullptr);
-    build_ghosts(t, gsigned, gd, gs);
+    build_ghosts(t, gsigned, gd, gs, gk);
     gsigned.clear();
     gsigned.shrink_to_fit();
     t->nghost = (int64_t)gd.size();
+    ghosts_by_voxel(t->nvox, gd, gk, cl, gout, gwave);
 
     // tensor divergence b_j = sum_i d_i D_ij with the one-sided rule of
     // new_11.cu:381-407 (tissue neighbours only)
@@ -344,6 +370,8 @@ lqt_status lqt_tissue_create(const lqt_geometry* geo, const lqt_params* params, 
         if ((s2 = upload(&t->cls, cl.data(), t->nvox))) return s2;
         if ((s2 = upload(&t->gdst, gd.data(), gd.size()))) return s2;
         if ((s2 = upload(&t->gsrc, gs.data(), gs.size()))) return s2;
+        if ((s2 = upload(&t->gout, gout.data(), gout.size()))) return s2;
+        if ((s2 = upload(&t->gwave, gwave.data(), gwave.size()))) return s2;
         LQT_HIP(hipMalloc(&t->grid[0], rs * npad));
         LQT_HIP(hipMalloc(&t->grid[1], rs * npad));
         LQT_HIP(hipMalloc(&t->r, rs * NR * t->pitch));
@@ -402,6 +430,8 @@ lqt_status lqt_tissue_destroy(lqt_tissue* t) {
     hipFree(t->drift);
     hipFree(t->gdst);
     hipFree(t->gsrc);
+    hipFree(t->gout);
+    hipFree(t->gwave);
     hipFree(t->consts);
     hipFree(t->stim_off);
     hipFree(t->s2);
@@ -496,40 +526,37 @@ lqt_status lqt_tissue_set_stimulus_file(lqt_tissue* t, const lqt_stim_protocol* 
 lqt_status lqt_tissue_step_async(lqt_tissue* t, int64_t nsteps, void* stream) {
     LQT_CHECK(t && nsteps >= 0, LQT_ERR_ARG, "lqt_tissue_step: bad argument");
     hipStream_t s = stream ? (hipStream_t)stream : t->stream;
-    const int64_t ntime = std::min<int64_t>(nsteps, 2048);
-    while ((int64_t)t->ev.size() < 4 * ntime) {
+    // every 16th step carries a pair of timing events (at most 128 pairs)
+    const int64_t ntime = std::min<int64_t>((nsteps + 15) / 16, 128);
+    while ((int64_t)t->ev.size() < 2 * ntime) {
         hipEvent_t e;
         LQT_HIP(hipEventCreate(&e));
         t->ev.push_back(e);
     }
     for (int64_t i = 0; i < nsteps; ++i) {
-        const bool timed = i < ntime;
+        const int64_t ti = i / 16;
+        const bool timed = (i % 16) == 0 && ti < ntime;
+        const double tc = t->time - floor(t->time / t->proto.bcl) * t->proto.bcl;
+        const int stim_on = t->nsites > 0 && tc >= t->off_min && tc < t->off_max + t->proto.duration;
+        const int s2_on = t->ns2 > 0 && t->proto.s2_duration > 0 && t->time >= t->proto.s2_time &&
+                          t->time < t->proto.s2_time + t->proto.s2_duration;
+        if (timed) LQT_HIP(hipEventRecord(t->ev[2 * ti], s));
         if (t->fp32) {
             TissueArgs<float> a;
             fill_args(t, a);
-            const double tc = t->time - floor(t->time / t->proto.bcl) * t->proto.bcl;
-            a.stim_on = t->nsites > 0 && tc >= t->off_min && tc < t->off_max + t->proto.duration;
+            a.stim_on = stim_on;
             a.tc = tc;
-            a.s2_on = t->ns2 > 0 && t->proto.s2_duration > 0 && t->time >= t->proto.s2_time &&
-                      t->time < t->proto.s2_time + t->proto.s2_duration;
-            if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i], s));
+            a.s2_on = s2_on;
             launch_tissue_step(true, t->aniso, t->nz == 1, t->variant, &a, t->nvox, s);
         } else {
             TissueArgs<double> a;
             fill_args(t, a);
-            const double tc = t->time - floor(t->time / t->proto.bcl) * t->proto.bcl;
-            a.stim_on = t->nsites > 0 && tc >= t->off_min && tc < t->off_max + t->proto.duration;
+            a.stim_on = stim_on;
             a.tc = tc;
-            a.s2_on = t->ns2 > 0 && t->proto.s2_duration > 0 && t->time >= t->proto.s2_time &&
-                      t->time < t->proto.s2_time + t->proto.s2_duration;
-            if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i], s));
+            a.s2_on = s2_on;
             launch_tissue_step(false, t->aniso, t->nz == 1, t->variant, &a, t->nvox, s);
         }
-        if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i + 1], s));
-        LQT_HIP(hipGetLastError());
-        if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i + 2], s));
-        launch_ghost_fill(t->fp32, t->grid[t->cur ^ 1], t->gdst, t->gsrc, t->nghost, s);
-        if (timed) LQT_HIP(hipEventRecord(t->ev[4 * i + 3], s));
+        if (timed) LQT_HIP(hipEventRecord(t->ev[2 * ti + 1], s));
         LQT_HIP(hipGetLastError());
         t->cur ^= 1;
         t->time += t->dt;  // GPU3D.cu:1840 double accumulator
@@ -537,16 +564,14 @@ lqt_status lqt_tissue_step_async(lqt_tissue* t, int64_t nsteps, void* stream) {
     }
     if (ntime > 0) {
         LQT_HIP(hipStreamSynchronize(s));
-        double a = 0, b = 0;
+        double a = 0;
         for (int64_t i = 0; i < ntime; ++i) {
-            float m1 = 0, m2 = 0;
-            LQT_HIP(hipEventElapsedTime(&m1, t->ev[4 * i], t->ev[4 * i + 1]));
-            LQT_HIP(hipEventElapsedTime(&m2, t->ev[4 * i + 2], t->ev[4 * i + 3]));
+            float m1 = 0;
+            LQT_HIP(hipEventElapsedTime(&m1, t->ev[2 * i], t->ev[2 * i + 1]));
             a += m1;
-            b += m2;
         }
         t->us_step = 1000.0 * a / ntime;
-        t->us_ghost = 1000.0 * b / ntime;
+        t->us_ghost = 0.0;  // the step kernel refreshes the ghosts itself
     }
     return LQT_OK;
 }
@@ -739,7 +764,9 @@ lqt_status lqt_tissue_bytes_per_step(const lqt_tissue* t, double* bytes) {
     // class byte and grid index, fibre + drift (anisotropic)
     double per = 2.0 * (rs + NR * rs + ND * 8.0) + 1.0 + 4.0;
     if (t->aniso) per += (t->nz == 1 ? 4.0 : 6.0) * rs;  // fibre + drift (x, y only in 2D)
-    *bytes = per * (double)t->nvox;
+    // plus the ghost refresh: destination index read and V written per
+    // ghost cell, one list offset per wavefront
+    *bytes = per * (double)t->nvox + (4.0 + rs) * (double)t->nghost + 4.0 * (double)((t->nvox + 63) / 64);
     return LQT_OK;
 }
 

@@ -57,6 +57,8 @@ struct TissueArgs {
     R s2_strength;
     const uint8_t* s2;
     unsigned long long* guard; // counts: V < -90, V > 200, non-finite
+    const int32_t* gout;       // ghost cells (padded index) listed by source voxel
+    const int32_t* gwave;      // first gout entry of each wavefront's voxels
 };
 
 // kernels_cells.hip

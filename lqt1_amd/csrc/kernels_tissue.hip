@@ -91,12 +91,30 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W))) void tis
 
     CellState<R> s;
     load_state(s, A.st, k);
-    const int c = A.cls[k];  // class, bit 7: V held at -100 mV (new_11.cu:1247-1248)
+    // cls: bits 0-1 class, bits 2-6 ghost cells mirroring this voxel,
+    // bit 7 V held at -100 mV (new_11.cu:1247-1248)
+    const int c = A.cls[k];
     const bool held = c & 0x80;
     R vn = tpord_update<R, false>(s, vc, c & 3, *A.consts, A.dt, ist, du, nullptr);
     if (held) vn = R(-100);
     store_state(s, A.st, k);
     *(R*)((char*)A.vnew + pb) = vn;
+
+    // the ghost mirrors of this voxel (GPU3D.cu:1566-1604) take the new V
+    // too, so the next step reads a complete grid without a refresh launch
+    const int gc = (c >> 2) & 31;
+    if (__ballot(gc != 0)) {
+        const int lane = threadIdx.x & 63;
+        int x = gc;  // inclusive prefix sum over the wavefront; lanes past
+                     // the last voxel have returned, and they are the top lanes
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        const int first = A.gwave[k >> 6] + x - gc;
+        for (int i = 0; i < gc; ++i) *(R*)((char*)A.vnew + (uint32_t)A.gout[first + i] * S) = vn;
+    }
 
     // V guard of GPU3D.cu:1796-1829, one atomic per wavefront.
     const bool lo = vn < R(-90) && !held, hi = vn > R(200), bad = !(vn == vn) || !(vn - vn == R(0));
