@@ -84,11 +84,11 @@ def cpu_baseline(geo, steps, threads):
 
 def measured_traffic(shape):
     """Bytes per launch the FP32 tissue kernel of launch shape (block, XCD
-    map, waves) moves beyond L2 on this workload, as the committed
+    map, waves, LDS staging) moves beyond L2 on this workload, as the committed
     rocprofv3 --pmc summary gives them (FETCH_SIZE and WRITE_SIZE per
     dispatch, each scaled by what the counter reads for a copy kernel of
     known size); None if the summary has no line for the shape."""
-    key = "traffic_bytes_per_launch %d %d %d:" % (shape[0], int(shape[1]), shape[2])
+    key = "traffic_bytes_per_launch %d %d %d %d:" % (shape[0], int(shape[1]), shape[2], int(shape[3]))
     try:
         with open(PROFILE_SUMMARY) as f:
             for line in f:
@@ -274,8 +274,8 @@ def main():
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 traffic_over_algorithmic=round(traffic / r32["bytes"], 4) if traffic else None,
                 copy_gbs=round(copy_gbs, 1), frac_of_copy=round(achieved / copy_gbs, 4),
-                kernel="tissue_step_kernel<float, true, false, %d, %s, %d>" % (
-                    r32["shape"][0], str(r32["shape"][1]).lower(), r32["shape"][2]),
+                kernel="tissue_step_kernel<float, true, false, %d, %s, %d, %s>" % (
+                    r32["shape"][0], str(r32["shape"][1]).lower(), r32["shape"][2], str(r32["shape"][3]).lower()),
                 kernel_us=round(r32["us_main"], 2), tune=r32["tune"],
                 bytes_per_launch=r32["bytes"])
 

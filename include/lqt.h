@@ -177,10 +177,11 @@ lqt_status lqt_tissue_save(lqt_tissue *t, const char *path);
 lqt_status lqt_tissue_load(lqt_tissue *t, const char *path);
 /* Launch shapes of the fused step kernel (workgroup size; whether
  * workgroups take voxel ranges in the XCD-aware order; waves per SIMD the
- * registers are allocated for).  Shape 0 is the default; a 2D sheet always
- * runs shape 0. */
+ * registers are allocated for; whether a wavefront whose voxels are one run
+ * along z stages its stencil columns in LDS).  Shape 0 is the default; a 2D
+ * sheet always runs shape 0. */
 lqt_status lqt_tissue_variants(int *count);
-lqt_status lqt_tissue_variant_info(int i, int *block, int *xcd_map, int *waves);
+lqt_status lqt_tissue_variant_info(int i, int *block, int *xcd_map, int *waves, int *lds_stage);
 lqt_status lqt_tissue_set_variant(lqt_tissue *t, int i);
 lqt_status lqt_tissue_get_variant(const lqt_tissue *t, int *i);
 /* Autotuner: runs every launch shape for trial_steps steps from the current

@@ -156,7 +156,7 @@ def lib():
     _sig(L, "lqt_strands_autotune", [_VP, C.c_int, P(C.c_int), _VP])
     _sig(L, "lqt_strands_set_mode", [_VP, C.c_int])
     _sig(L, "lqt_tissue_variants", [P(C.c_int)])
-    _sig(L, "lqt_tissue_variant_info", [C.c_int, P(C.c_int), P(C.c_int), P(C.c_int)])
+    _sig(L, "lqt_tissue_variant_info", [C.c_int, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int)])
     _sig(L, "lqt_tissue_set_variant", [_VP, C.c_int])
     _sig(L, "lqt_tissue_get_variant", [_VP, P(C.c_int)])
     _sig(L, "lqt_tissue_autotune", [_VP, C.c_int, P(C.c_int), _VP])

@@ -669,11 +669,12 @@ lqt_status lqt_tissue_variants(int* count) {
     return LQT_OK;
 }
 
-lqt_status lqt_tissue_variant_info(int i, int* block, int* xcd_map, int* waves) {
+lqt_status lqt_tissue_variant_info(int i, int* block, int* xcd_map, int* waves, int* lds_stage) {
     LQT_CHECK(i >= 0 && i < kNumTissueVariants, LQT_ERR_ARG, "lqt_tissue_variant_info: no variant %d", i);
     if (block) *block = kTissueVariants[i].block;
     if (xcd_map) *xcd_map = kTissueVariants[i].xcd;
     if (waves) *waves = kTissueVariants[i].waves;
+    if (lds_stage) *lds_stage = kTissueVariants[i].lds;
     return LQT_OK;
 }
 

@@ -71,12 +71,12 @@ void launch_cells_pace(bool fp32, int n, void* r, double* d, int64_t pitch, void
 
 // kernels_tissue.hip
 // Launch shapes of the fused tissue step (block size, XCD-aware map, waves
-// per SIMD the registers are allocated for); entry 0 is the default (the
-// fastest on the 6.4M-voxel ventricle).
+// per SIMD the registers are allocated for, stencil staged in LDS); entry 0
+// is the default.
 struct TissueVariant {
-    int block, xcd, waves;
+    int block, xcd, waves, lds;
 };
-constexpr int kNumTissueVariants = 6;
+constexpr int kNumTissueVariants = 8;
 extern const TissueVariant kTissueVariants[kNumTissueVariants];
 void launch_tissue_step(bool fp32, bool aniso, bool flat, int variant, const void* args, int64_t nvox,
                         hipStream_t st);

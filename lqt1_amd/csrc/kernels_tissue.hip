@@ -30,9 +30,12 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
 // workgroup; XCD: the workgroup -> voxel-range map of xcd_tile (otherwise
 // workgroup b takes voxels b*B .. b*B+B-1); W: waves per SIMD the register
 // allocator targets (2 holds the FP32 anisotropic state in 192 registers;
-// FP64 spills at 2 and not at 1).
-template <class R, bool ANISO, bool FLAT, int B, bool XCD, int W>
+// FP64 spills at 2 and not at 1); LDS: a wavefront whose 64 voxels are one
+// run along z stages the 9 columns around the run (z0-1 .. z0+64) in LDS
+// and reads its 19-point stencil from there (3D anisotropic path).
+template <class R, bool ANISO, bool FLAT, int B, bool XCD, int W, bool LDS>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W))) void tissue_step_kernel(TissueArgs<R> A) {
+    __shared__ R tile[LDS ? B / 64 : 1][9][66];
     // 32-bit byte offsets from uniform bases let the loads share address
     // registers (voxel and padded-grid counts are bounded on the host)
     const int64_t blk = XCD ? xcd_tile(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
@@ -43,9 +46,41 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W))) void tis
     const R* __restrict__ vo = A.vold;
     const uint32_t sx = (uint32_t)A.sx * S, sy = (uint32_t)A.sy * S;
     auto V = [&](uint32_t off) -> R { return *(const R*)((const char*)vo + off); };
-    const R vc = V(pb);
-    const R vxp = V(pb + sx), vxm = V(pb - sx);
-    const R vyp = V(pb + sy), vym = V(pb - sy);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    bool staged = false;
+    if (LDS && ANISO && !FLAT) {
+        // all 64 lanes present (every staged read is then the neighbour of a
+        // tissue voxel or of its z-neighbour) and consecutive along z
+        const uint32_t pb0 = __shfl(pb, 0, 64);
+        const bool ok = pb == pb0 + (uint32_t)lane * S;
+        staged = __ballot(1) == ~0ull && __ballot(!ok) == 0ull;
+        if (staged) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                const int dx = c / 3 - 1, dy = c % 3 - 1;
+                const uint32_t base = pb0 + (uint32_t)(dx * (int)sx + dy * (int)sy);
+                tile[wv][c][lane + 1] = V(base + (uint32_t)lane * S);
+            }
+            if (lane < 18) {
+                const int c = lane >> 1, dx = c / 3 - 1, dy = c % 3 - 1;
+                const uint32_t base = pb0 + (uint32_t)(dx * (int)sx + dy * (int)sy);
+                if (lane & 1) tile[wv][c][65] = V(base + 64u * S);
+                else tile[wv][c][0] = V(base - S);
+            }
+            // the wavefront reads what its other lanes wrote
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    // neighbour (dx, dy, dz) of the voxel
+    auto G = [&](int dx, int dy, int dz) -> R {
+        if (LDS && staged) return tile[wv][(dx + 1) * 3 + (dy + 1)][lane + 1 + dz];
+        return V(pb + (uint32_t)(dx * (int)sx + dy * (int)sy + dz * (int)S));
+    };
+    const R vc = G(0, 0, 0);
+    const R vxp = G(1, 0, 0), vxm = G(-1, 0, 0);
+    const R vyp = G(0, 1, 0), vym = G(0, -1, 0);
     R du;
     if (FLAT) {
         const R vxx = (vxm + vxp - R(2) * vc) * A.inv_h2;
@@ -62,13 +97,13 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W))) void tis
             du = A.dcross * (vxx + vyy);
         }
     } else if (ANISO) {
-        const R vzp = V(pb + S), vzm = V(pb - S);
+        const R vzp = G(0, 0, 1), vzm = G(0, 0, -1);
         const R vxx = (vxm + vxp - R(2) * vc) * A.inv_h2;
         const R vyy = (vym + vyp - R(2) * vc) * A.inv_h2;
         const R vzz = (vzm + vzp - R(2) * vc) * A.inv_h2;
-        const R vxy = (V(pb + sx + sy) + V(pb - sx - sy) - V(pb - sx + sy) - V(pb + sx - sy)) * A.inv_4h2;
-        const R vxz = (V(pb + sx + S) + V(pb - sx - S) - V(pb - sx + S) - V(pb + sx - S)) * A.inv_4h2;
-        const R vyz = (V(pb + sy + S) + V(pb - sy - S) - V(pb - sy + S) - V(pb + sy - S)) * A.inv_4h2;
+        const R vxy = (G(1, 1, 0) + G(-1, -1, 0) - G(-1, 1, 0) - G(1, -1, 0)) * A.inv_4h2;
+        const R vxz = (G(1, 0, 1) + G(-1, 0, -1) - G(-1, 0, 1) - G(1, 0, -1)) * A.inv_4h2;
+        const R vyz = (G(0, 1, 1) + G(0, -1, -1) - G(0, -1, 1) - G(0, 1, -1)) * A.inv_4h2;
         const uint32_t ok = k * S;
         const R f0 = ld_nt(A.fib, ok), f1 = ld_nt(A.fib + A.pitch, ok), f2 = ld_nt(A.fib + 2 * A.pitch, ok);
         const R b0 = ld_nt(A.drift, ok), b1 = ld_nt(A.drift + A.pitch, ok), b2 = ld_nt(A.drift + 2 * A.pitch, ok);
@@ -154,12 +189,13 @@ void launch_copy(void* dst, const void* src, int64_t bytes, hipStream_t st) {
 }
 
 const TissueVariant kTissueVariants[kNumTissueVariants] = {
-    {256, 0, 2}, {256, 1, 2}, {128, 1, 2}, {512, 1, 2}, {256, 1, 1}, {128, 1, 1},
+    {256, 0, 2, 0}, {256, 1, 2, 0}, {128, 1, 2, 0}, {512, 1, 2, 0},
+    {256, 1, 1, 0}, {128, 1, 1, 0}, {256, 0, 2, 1}, {256, 1, 2, 1},
 };
 
-template <class R, bool ANISO, bool FLAT, int B, bool XCD, int W>
+template <class R, bool ANISO, bool FLAT, int B, bool XCD, int W, bool LDS = false>
 static void launch_v(const TissueArgs<R>& a, int64_t nvox, hipStream_t st) {
-    tissue_step_kernel<R, ANISO, FLAT, B, XCD, W><<<(nvox + B - 1) / B, B, 0, st>>>(a);
+    tissue_step_kernel<R, ANISO, FLAT, B, XCD, W, LDS><<<(nvox + B - 1) / B, B, 0, st>>>(a);
 }
 
 template <class R, bool ANISO>
@@ -174,6 +210,8 @@ static void launch_a(bool flat, int variant, const TissueArgs<R>& a, int64_t nvo
         case 3: launch_v<R, ANISO, false, 512, true, 2>(a, nvox, st); break;
         case 4: launch_v<R, ANISO, false, 256, true, 1>(a, nvox, st); break;
         case 5: launch_v<R, ANISO, false, 128, true, 1>(a, nvox, st); break;
+        case 6: launch_v<R, ANISO, false, 256, false, 2, true>(a, nvox, st); break;
+        case 7: launch_v<R, ANISO, false, 256, true, 2, true>(a, nvox, st); break;
         default: launch_v<R, ANISO, false, 256, false, 2>(a, nvox, st); break;
     }
 }

@@ -80,14 +80,14 @@ class Tissue:
 
     @staticmethod
     def variants():
-        """Launch shapes of the fused step kernel: [(block, xcd_map, waves), ...]."""
+        """Launch shapes of the fused step kernel: [(block, xcd_map, waves, lds_stage), ...]."""
         n = C.c_int()
         L.check(L.lib().lqt_tissue_variants(C.byref(n)))
         out = []
         for i in range(n.value):
-            b, x, w = C.c_int(), C.c_int(), C.c_int()
-            L.check(L.lib().lqt_tissue_variant_info(i, C.byref(b), C.byref(x), C.byref(w)))
-            out.append((b.value, bool(x.value), w.value))
+            b, x, w, s = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+            L.check(L.lib().lqt_tissue_variant_info(i, C.byref(b), C.byref(x), C.byref(w), C.byref(s)))
+            out.append((b.value, bool(x.value), w.value, bool(s.value)))
         return out
 
     def autotune(self, trial_steps=20):

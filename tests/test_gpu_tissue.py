@@ -150,20 +150,38 @@ def test_launch_shapes_agree_and_autotune(lqt, prec):
     P = lqt.FP32 if prec == "fp32" else lqt.FP64
     lab, fib, sites = _blob(3, (16, 14, 12))
     shapes = lqt.Tissue.variants()
-    assert len(shapes) >= 4 and shapes[0] == (256, False, 2)
-    ref = None
-    for i in range(len(shapes)):
-        T = lqt.Tissue(lab, fib, 0.2, 0.154, 0.0385, lqt.preset("cpu_default"), P, 0.02)
-        T.set_stimulus(sites, None, strength=-80.0)
-        T.set_variant(i)
-        T.step(120)
-        out = (T.v(), T.state(T.nvox // 3))
-        if ref is None:
-            ref = out
-        else:
-            assert np.array_equal(out[0], ref[0]), shapes[i]
-            assert np.array_equal(out[1], ref[1]), shapes[i]
-        T.close()
+    assert len(shapes) >= 4 and shapes[0] == (256, False, 2, False)
+    assert any(s[3] for s in shapes)
+    # a slab of long z-runs (260 voxels per column), so that most
+    # wavefronts are one run and the LDS-staged shapes take their LDS path
+    slab = np.full((6, 6, 262), 255, np.uint8)
+    slab[1:5, 1:5, 1:261] = 3
+    slab[2, 2, 100] = 255  # a hole: ghosts inside the slab
+    zz = np.arange(262)[None, None, :]
+    sfib = np.zeros(slab.shape + (3,), np.float32)
+    sfib[..., 0] = np.cos(zz / 40.0)
+    sfib[..., 2] = np.sin(zz / 40.0)
+    ssites = np.flatnonzero((slab != 255) & (zz < 4))
+    for geo in ((lab, fib, sites), (slab, sfib, ssites)):
+        ref = None
+        for i in range(len(shapes)):
+            T = lqt.Tissue(geo[0], geo[1], 0.2, 0.154, 0.0385, lqt.preset("cpu_default"), P, 0.02)
+            T.set_stimulus(geo[2], None, strength=-80.0)
+            T.set_variant(i)
+            T.step(120)
+            out = (T.v(), T.state(T.nvox // 3))
+            if ref is None:
+                ref = out
+                assert np.ptp(out[0]) > 1.0
+            else:
+                assert np.array_equal(out[0], ref[0]), shapes[i]
+                assert np.array_equal(out[1], ref[1]), shapes[i]
+            T.close()
+    T = lqt.Tissue(lab, fib, 0.2, 0.154, 0.0385, lqt.preset("cpu_default"), P, 0.02)
+    T.set_stimulus(sites, None, strength=-80.0)
+    T.step(120)
+    ref = (T.v(), T.state(T.nvox // 3))
+    T.close()
     T = lqt.Tissue(lab, fib, 0.2, 0.154, 0.0385, lqt.preset("cpu_default"), P, 0.02)
     T.set_stimulus(sites, None, strength=-80.0)
     T.step(60)
