@@ -82,13 +82,25 @@ def cpu_baseline(geo, steps, threads):
     return nvox * steps / wall, wall
 
 
+def shape_key(shape):
+    """Launch shape (block, XCD map, waves, LDS staging) as the four
+    integers the profile summary lists."""
+    return " ".join(str(int(x)) for x in shape)
+
+
+def kernel_name(shape, real="float"):
+    """Name rocprofv3 reports for the anisotropic 3D step kernel of a launch shape."""
+    b, x, w, lds = shape
+    return "tissue_step_kernel<%s, true, false, %d, %s, %d, %s>" % (real, b, str(x).lower(), w, str(lds).lower())
+
+
 def measured_traffic(shape):
     """Bytes per launch the FP32 tissue kernel of launch shape (block, XCD
-    map, waves, LDS staging) moves beyond L2 on this workload, as the committed
-    rocprofv3 --pmc summary gives them (FETCH_SIZE and WRITE_SIZE per
-    dispatch, each scaled by what the counter reads for a copy kernel of
+    map, waves, LDS staging) moves beyond L2 on this workload, as the
+    committed rocprofv3 --pmc summary gives them (FETCH_SIZE and WRITE_SIZE
+    per dispatch, each scaled by what the counter reads for a copy kernel of
     known size); None if the summary has no line for the shape."""
-    key = "traffic_bytes_per_launch %d %d %d %d:" % (shape[0], int(shape[1]), shape[2], int(shape[3]))
+    key = "traffic_bytes_per_launch %s:" % shape_key(shape)
     try:
         with open(PROFILE_SUMMARY) as f:
             for line in f:
@@ -274,8 +286,7 @@ def main():
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 traffic_over_algorithmic=round(traffic / r32["bytes"], 4) if traffic else None,
                 copy_gbs=round(copy_gbs, 1), frac_of_copy=round(achieved / copy_gbs, 4),
-                kernel="tissue_step_kernel<float, true, false, %d, %s, %d, %s>" % (
-                    r32["shape"][0], str(r32["shape"][1]).lower(), r32["shape"][2], str(r32["shape"][3]).lower()),
+                kernel=kernel_name(r32["shape"]),
                 kernel_us=round(r32["us_main"], 2), tune=r32["tune"],
                 bytes_per_launch=r32["bytes"])
 

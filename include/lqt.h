@@ -179,7 +179,8 @@ lqt_status lqt_tissue_load(lqt_tissue *t, const char *path);
  * workgroups take voxel ranges in the XCD-aware order; waves per SIMD the
  * registers are allocated for; whether a wavefront whose voxels are one run
  * along z stages its stencil columns in LDS).  Shape 0 is the default; a 2D
- * sheet always runs shape 0. */
+ * sheet always runs shape 0.  No counterpart in the reference, which
+ * launches <<<ceil(n/512), 512>>> (GPU3D.cu:1757-1790). */
 lqt_status lqt_tissue_variants(int *count);
 lqt_status lqt_tissue_variant_info(int i, int *block, int *xcd_map, int *waves, int *lds_stage);
 lqt_status lqt_tissue_set_variant(lqt_tissue *t, int i);

@@ -36,7 +36,12 @@ SOURCES = [
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-value", "-Wno-unused-result",
           "-I" + os.path.join(HERE, "..", "include")]
-DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# Multiply-adds are fused only within one source expression
+# (-ffp-contract=on), never across statements: HIP's default lets the
+# backend fuse by context, so the same update inlined into two kernels could
+# round differently, and every launch shape of a kernel must give the same
+# bits.
+DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=on"]
 
 
 def _compile(src: str) -> str:

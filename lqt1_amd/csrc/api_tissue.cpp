@@ -32,8 +32,7 @@ struct lqt_tissue {
     int cur = 0;
     void* r = nullptr;
     double* d = nullptr;
-    void* fib = nullptr;
-    void* drift = nullptr;
+    void* fd = nullptr;  // fibre and drift, chunk-major (6 slots)
     int32_t* gdst = nullptr;   // ghost cells (padded index) and their sources,
     int32_t* gsrc = nullptr;   // for refreshes outside the step kernel
     int32_t* gout = nullptr;   // ghost cells listed by the voxel they mirror
@@ -56,6 +55,7 @@ struct lqt_tissue {
 };
 
 static size_t rsz(const lqt_tissue* t) { return t->fp32 ? sizeof(float) : sizeof(double); }
+
 
 // Ghost map of GPU3D.cu:1487-1604: every non-tissue cell with a tissue cell
 // among its 26 neighbours mirrors the tissue neighbour closest to the
@@ -232,15 +232,14 @@ static void fill_args(lqt_tissue* t, TissueArgs<R>& a) {
     a.vnew = (R*)t->grid[t->cur ^ 1];
     a.sx = t->py * t->pz;
     a.sy = t->pz;
-    a.fib = (const R*)t->fib;
-    a.drift = (const R*)t->drift;
-    a.pitch = t->pitch;
+    a.fd = (const R*)t->fd;
     a.dcross = (R)t->d_cross;
     a.ddiff = (R)(t->d_fibre - t->d_cross);
     a.inv_h2 = (R)(1.0 / (t->dx * t->dx));
     a.inv_4h2 = (R)(1.0 / (4.0 * t->dx * t->dx));
     a.inv_2h = (R)(1.0 / (2.0 * t->dx));
-    a.st = SoA<R>{(R*)t->r, t->d, t->pitch};
+    a.sr = (R*)t->r;
+    a.sd = t->d;
     a.consts = (const Consts<R>*)t->consts;
     a.dt = (R)t->dt;
     a.stim_on = 0;
@@ -299,6 +298,7 @@ lqt_status lqt_tissue_create(const lqt_geometry* geo, const lqt_params* params, 
         delete t;
         return set_error(LQT_ERR_ARG, "lqt_tissue_create: geometry has no tissue voxel");
     }
+    // voxel slots per array: whole 64-voxel chunks, rounded up to 256
     t->pitch = (t->nvox + 255) / 256 * 256;
     const int64_t SX = (int64_t)t->ny * t->nz, SY = t->nz;
     t->h_vpos.resize(t->nvox);
@@ -379,21 +379,16 @@ lqt_status lqt_tissue_create(const lqt_geometry* geo, const lqt_params* params, 
         LQT_HIP(hipMalloc((void**)&t->guard, sizeof(unsigned long long) * 3));
         LQT_HIP(hipMemset(t->guard, 0, sizeof(unsigned long long) * 3));
         if (t->aniso) {
-            // SoA: component j of voxel k at j*pitch + k
-            std::vector<unsigned char> fb(rs * 3 * t->pitch, 0), db(rs * 3 * t->pitch, 0);
+            // chunk-major: slots 0-2 fibre f, 3-5 divergence of D
+            std::vector<unsigned char> fb(rs * 6 * t->pitch, 0);
             for (int64_t k = 0; k < t->nvox; ++k)
-                for (int j = 0; j < 3; ++j) {
-                    const int64_t idx = (int64_t)j * t->pitch + k;
-                    if (t->fp32) {
-                        ((float*)fb.data())[idx] = (float)fibre_h[3 * k + j];
-                        ((float*)db.data())[idx] = (float)drift_h[3 * k + j];
-                    } else {
-                        ((double*)fb.data())[idx] = fibre_h[3 * k + j];
-                        ((double*)db.data())[idx] = drift_h[3 * k + j];
-                    }
+                for (int j = 0; j < 6; ++j) {
+                    const int64_t idx = chunk_slot(k, j, 6);
+                    const double x = j < 3 ? fibre_h[3 * k + j] : drift_h[3 * k + j - 3];
+                    if (t->fp32) ((float*)fb.data())[idx] = (float)x;
+                    else ((double*)fb.data())[idx] = x;
                 }
-            if ((s2 = upload((unsigned char**)&t->fib, fb.data(), fb.size()))) return s2;
-            if ((s2 = upload((unsigned char**)&t->drift, db.data(), db.size()))) return s2;
+            if ((s2 = upload((unsigned char**)&t->fd, fb.data(), fb.size()))) return s2;
         }
         const Consts<double> cd = derive_consts(*params);
         if (t->fp32) {
@@ -426,8 +421,7 @@ lqt_status lqt_tissue_destroy(lqt_tissue* t) {
     hipFree(t->grid[1]);
     hipFree(t->r);
     hipFree(t->d);
-    hipFree(t->fib);
-    hipFree(t->drift);
+    hipFree(t->fd);
     hipFree(t->gdst);
     hipFree(t->gsrc);
     hipFree(t->gout);
@@ -618,11 +612,10 @@ lqt_status lqt_tissue_read_state(lqt_tissue* t, int64_t k, double* out) {
             LQT_HIP(hipMemcpy(buf, (unsigned char*)t->grid[t->cur] + rs * t->h_vpos[k], rs, hipMemcpyDeviceToHost));
             val = t->fp32 ? (double)*(float*)buf : *(double*)buf;
         } else if (m.kind == 1) {
-            LQT_HIP(hipMemcpy(buf, (unsigned char*)t->r + rs * ((int64_t)m.slot * t->pitch + k), rs,
-                              hipMemcpyDeviceToHost));
+            LQT_HIP(hipMemcpy(buf, (unsigned char*)t->r + rs * chunk_slot(k, m.slot, NR), rs, hipMemcpyDeviceToHost));
             val = t->fp32 ? (double)*(float*)buf : *(double*)buf;
         } else if (m.kind == 2) {
-            LQT_HIP(hipMemcpy(&val, t->d + (int64_t)m.slot * t->pitch + k, sizeof(double), hipMemcpyDeviceToHost));
+            LQT_HIP(hipMemcpy(&val, t->d + chunk_slot(k, m.slot, ND), sizeof(double), hipMemcpyDeviceToHost));
         }
         out[j] = val;
     }
@@ -641,10 +634,9 @@ lqt_status lqt_tissue_write_state(lqt_tissue* t, int64_t k, const double* in) {
         if (m.kind == 0) {
             LQT_HIP(hipMemcpy((unsigned char*)t->grid[t->cur] + rs * t->h_vpos[k], buf, rs, hipMemcpyHostToDevice));
         } else if (m.kind == 1) {
-            LQT_HIP(hipMemcpy((unsigned char*)t->r + rs * ((int64_t)m.slot * t->pitch + k), buf, rs,
-                              hipMemcpyHostToDevice));
+            LQT_HIP(hipMemcpy((unsigned char*)t->r + rs * chunk_slot(k, m.slot, NR), buf, rs, hipMemcpyHostToDevice));
         } else if (m.kind == 2) {
-            LQT_HIP(hipMemcpy(t->d + (int64_t)m.slot * t->pitch + k, &in[j], sizeof(double), hipMemcpyHostToDevice));
+            LQT_HIP(hipMemcpy(t->d + chunk_slot(k, m.slot, ND), &in[j], sizeof(double), hipMemcpyHostToDevice));
         }
     }
     // refresh the ghost mirrors of the current grid
@@ -790,11 +782,11 @@ static lqt_status tissue_upload_state(lqt_tissue* t, const std::vector<double>& 
                 if (t->fp32) ((float*)g.data())[t->h_vpos[k]] = (float)val;
                 else ((double*)g.data())[t->h_vpos[k]] = val;
             } else if (m.kind == 1) {
-                const int64_t idx = (int64_t)m.slot * t->pitch + k;
+                const int64_t idx = chunk_slot(k, m.slot, NR);
                 if (t->fp32) ((float*)r.data())[idx] = (float)val;
                 else ((double*)r.data())[idx] = val;
             } else if (m.kind == 2) {
-                d[(size_t)m.slot * t->pitch + k] = val;
+                d[(size_t)chunk_slot(k, m.slot, ND)] = val;
             }
         }
     }
@@ -889,8 +881,9 @@ lqt_status lqt_tissue_write_vtk(lqt_tissue* t, const char* path, int stride) {
     return LQT_OK;
 }
 
-// Checkpoint: header, time, step, then every voxel's 40-state vector.
-static const char kMagic[8] = {'L', 'Q', 'T', 'T', 'I', 'S', '1', 0};
+// Checkpoint: magic, header (dims, voxels, precision, step), time, then the
+// state arrays as stored (chunk-major) and the current V grid.
+static const char kMagic[8] = {'L', 'Q', 'T', 'T', 'I', 'S', '2', 0};
 
 lqt_status lqt_tissue_save(lqt_tissue* t, const char* path) {
     LQT_CHECK(t && path, LQT_ERR_ARG, "lqt_tissue_save: bad argument");

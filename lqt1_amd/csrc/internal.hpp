@@ -39,12 +39,11 @@ struct TissueArgs {
     const R* vold;             // dense padded V grid, step n
     R* vnew;                   // dense padded V grid, step n+1
     int32_t sx, sy;            // padded strides of x and y (z stride is 1)
-    const R* fib;              // SoA fibre components, 3 * pitch
-    const R* drift;            // SoA divergence of D, 3 * pitch
-    int64_t pitch;
+    const R* fd;               // chunk-major, 6 slots: fibre f (3), divergence of D (3)
     R dcross, ddiff;           // D_cross, D_fibre - D_cross
     R inv_h2, inv_4h2, inv_2h; // 1/dx^2, 1/(4dx^2), 1/(2dx)
-    SoA<R> st;
+    R* sr;                     // state, chunk-major: NR slots of R per chunk
+    double* sd;                // state, chunk-major: ND slots of double per chunk
     const Consts<R>* consts;
     R dt;
     // stimulus of this step

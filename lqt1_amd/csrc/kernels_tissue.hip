@@ -8,11 +8,109 @@
 // so every neighbour read is a fixed offset from the voxel's grid index:
 // consecutive lanes read consecutive z, which keeps all 19 stencil reads
 // coalesced.  Ghost cells (the reference's mirror map, GPU3D.cu:1566-1604)
-// are refreshed after each step by ghost_fill_kernel; all other non-tissue
-// cells hold -90 mV (GPU3D.cu:305).
+// take the new V of the voxel they mirror inside the step kernel itself;
+// ghost_fill_kernel only refreshes them after the host writes state.  All
+// other non-tissue cells hold -90 mV (GPU3D.cu:305).
 #include "internal.hpp"
 
 namespace lqt {
+
+// Fibre direction f and divergence of D of one voxel (chunk-major slots
+// 0-2 and 3-5 of the static voxel data).
+template <class R>
+struct Fibre {
+    R f0, f1, f2, b0, b1, b2;
+};
+
+template <class R, bool ANISO, bool FLAT>
+__device__ __forceinline__ void load_fibre(Fibre<R>& F, const TissueArgs<R>& A, uint32_t k) {
+    if (!ANISO) return;
+    const R* b = A.fd + (size_t)(k >> 6) * (6 * kChunk) + (k & 63);
+    constexpr uint32_t w = kChunk * sizeof(R);
+    F.f0 = ld_nt(b, 0);
+    F.f1 = ld_nt(b, w);
+    F.b0 = ld_nt(b, 3 * w);
+    F.b1 = ld_nt(b, 4 * w);
+    if (!FLAT) {
+        F.f2 = ld_nt(b, 2 * w);
+        F.b2 = ld_nt(b, 5 * w);
+    }
+}
+
+// du = div(D grad V) at one voxel from its neighbours G(dx, dy, dz)
+// (new_11.cu:352-417 generalised to 3D; 2D sheet without z terms;
+// isotropic: GPU3D.cu:370-385).
+template <class R, bool ANISO, bool FLAT, class NB>
+__device__ __forceinline__ R diffusion(const TissueArgs<R>& A, const Fibre<R>& F, R vc, NB G) {
+    const R vxp = G(1, 0, 0), vxm = G(-1, 0, 0);
+    const R vyp = G(0, 1, 0), vym = G(0, -1, 0);
+    if (FLAT) {
+        const R vxx = (vxm + vxp - R(2) * vc) * A.inv_h2;
+        const R vyy = (vym + vyp - R(2) * vc) * A.inv_h2;
+        if (!ANISO) return A.dcross * (vxx + vyy);
+        const R vxy = (G(1, 1, 0) + G(-1, -1, 0) - G(-1, 1, 0) - G(1, -1, 0)) * A.inv_4h2;
+        const R dd = A.ddiff, dc = A.dcross;
+        return (dc + dd * F.f0 * F.f0) * vxx + (dc + dd * F.f1 * F.f1) * vyy + R(2) * dd * F.f0 * F.f1 * vxy +
+               (F.b0 * (vxp - vxm) + F.b1 * (vyp - vym)) * A.inv_2h;
+    }
+    const R vzp = G(0, 0, 1), vzm = G(0, 0, -1);
+    if (!ANISO)
+        return A.dcross * ((vxm + vxp - R(2) * vc) + (vym + vyp - R(2) * vc) + (vzm + vzp - R(2) * vc)) * A.inv_h2;
+    const R vxx = (vxm + vxp - R(2) * vc) * A.inv_h2;
+    const R vyy = (vym + vyp - R(2) * vc) * A.inv_h2;
+    const R vzz = (vzm + vzp - R(2) * vc) * A.inv_h2;
+    const R vxy = (G(1, 1, 0) + G(-1, -1, 0) - G(-1, 1, 0) - G(1, -1, 0)) * A.inv_4h2;
+    const R vxz = (G(1, 0, 1) + G(-1, 0, -1) - G(-1, 0, 1) - G(1, 0, -1)) * A.inv_4h2;
+    const R vyz = (G(0, 1, 1) + G(0, -1, -1) - G(0, -1, 1) - G(0, 1, -1)) * A.inv_4h2;
+    const R dd = A.ddiff, dc = A.dcross;
+    // D = Dc*I + (Df-Dc) f f^T ; du = sum_ij D_ij d_i d_j V + sum_j b_j d_j V
+    return (dc + dd * F.f0 * F.f0) * vxx + (dc + dd * F.f1 * F.f1) * vyy + (dc + dd * F.f2 * F.f2) * vzz +
+           R(2) * dd * (F.f0 * F.f1 * vxy + F.f0 * F.f2 * vxz + F.f1 * F.f2 * vyz) +
+           (F.b0 * (vxp - vxm) + F.b1 * (vyp - vym) + F.b2 * (vzp - vzm)) * A.inv_2h;
+}
+
+// Stimulus current of voxel k on this step (GPU3D.cu:387-422).
+template <class R>
+__device__ __forceinline__ R stimulus(const TissueArgs<R>& A, uint32_t k) {
+    R ist = R(0);
+    if (A.stim_on) {
+        const float off = A.stim_off[k];
+        if (A.tc >= (double)off && A.tc < (double)off + A.stim_dur) ist = A.stim_strength;
+    }
+    if (A.s2_on && A.s2[k]) ist = A.s2_strength;
+    return ist;
+}
+
+// V guard of GPU3D.cu:1796-1829, one atomic per wavefront.  Lanes with
+// on == false take no part.
+template <class R>
+__device__ __forceinline__ void guard_v(const TissueArgs<R>& A, R vn, bool held, bool on) {
+    const bool lo = on && vn < R(-90) && !held, hi = on && vn > R(200);
+    const bool bad = on && (!(vn == vn) || !(vn - vn == R(0)));
+    const unsigned long long mlo = __ballot(lo), mhi = __ballot(hi), mbad = __ballot(bad);
+    if ((mlo | mhi | mbad) && (threadIdx.x & 63) == (__ffsll((long long)(mlo | mhi | mbad)) - 1)) {
+        if (mlo) atomicAdd(&A.guard[0], (unsigned long long)__popcll(mlo));
+        if (mhi) atomicAdd(&A.guard[1], (unsigned long long)__popcll(mhi));
+        if (mbad) atomicAdd(&A.guard[2], (unsigned long long)__popcll(mbad));
+    }
+}
+
+// The ghost mirrors of this wavefront's voxels (GPU3D.cu:1566-1604) take
+// their new V: gc ghost cells per lane (0 on lanes without a voxel), listed
+// from gout[gwave[chunk]] on in lane order.
+template <class R>
+__device__ __forceinline__ void write_ghosts(const TissueArgs<R>& A, uint32_t chunk, int gc, R vn) {
+    if (!__ballot(gc != 0)) return;
+    const int lane = threadIdx.x & 63;
+    int x = gc;  // inclusive prefix sum over the wavefront
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    const int first = A.gwave[chunk] + x - gc;
+    for (int i = 0; i < gc; ++i) *(R*)((char*)A.vnew + (uint32_t)A.gout[first + i] * (uint32_t)sizeof(R)) = vn;
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs of the MI355X.
 // This remap gives each XCD one contiguous range of tiles so that the
@@ -28,9 +126,10 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
 
 // FLAT: a 2D sheet (nz == 1, new_11.cu:352-417), no z terms.  B lanes per
 // workgroup; XCD: the workgroup -> voxel-range map of xcd_tile (otherwise
-// workgroup b takes voxels b*B .. b*B+B-1); W: waves per SIMD the register
-// allocator targets (2 holds the FP32 anisotropic state in 192 registers;
-// FP64 spills at 2 and not at 1); LDS: a wavefront whose 64 voxels are one
+// workgroup b takes voxels b*B .. b*B+B-1); W: at least W waves per SIMD
+// for the register allocator (the FP32 anisotropic step needs 155
+// registers, so three waves fit either way; FP64 spills at 2 and not at 1);
+// LDS: a wavefront whose 64 voxels are one
 // run along z stages the 9 columns around the run (z0-1 .. z0+64) in LDS
 // and reads its 19-point stencil from there (3D anisotropic path).
 template <class R, bool ANISO, bool FLAT, int B, bool XCD, int W, bool LDS>
@@ -78,87 +177,27 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W))) void tis
         if (LDS && staged) return tile[wv][(dx + 1) * 3 + (dy + 1)][lane + 1 + dz];
         return V(pb + (uint32_t)(dx * (int)sx + dy * (int)sy + dz * (int)S));
     };
+    Fibre<R> F;
+    load_fibre<R, ANISO, FLAT>(F, A, k);
     const R vc = G(0, 0, 0);
-    const R vxp = G(1, 0, 0), vxm = G(-1, 0, 0);
-    const R vyp = G(0, 1, 0), vym = G(0, -1, 0);
-    R du;
-    if (FLAT) {
-        const R vxx = (vxm + vxp - R(2) * vc) * A.inv_h2;
-        const R vyy = (vym + vyp - R(2) * vc) * A.inv_h2;
-        if (ANISO) {
-            const R vxy = (V(pb + sx + sy) + V(pb - sx - sy) - V(pb - sx + sy) - V(pb + sx - sy)) * A.inv_4h2;
-            const uint32_t ok = k * S;
-            const R f0 = ld_nt(A.fib, ok), f1 = ld_nt(A.fib + A.pitch, ok);
-            const R b0 = ld_nt(A.drift, ok), b1 = ld_nt(A.drift + A.pitch, ok);
-            const R dd = A.ddiff, dc = A.dcross;
-            du = (dc + dd * f0 * f0) * vxx + (dc + dd * f1 * f1) * vyy + R(2) * dd * f0 * f1 * vxy +
-                 (b0 * (vxp - vxm) + b1 * (vyp - vym)) * A.inv_2h;
-        } else {
-            du = A.dcross * (vxx + vyy);
-        }
-    } else if (ANISO) {
-        const R vzp = G(0, 0, 1), vzm = G(0, 0, -1);
-        const R vxx = (vxm + vxp - R(2) * vc) * A.inv_h2;
-        const R vyy = (vym + vyp - R(2) * vc) * A.inv_h2;
-        const R vzz = (vzm + vzp - R(2) * vc) * A.inv_h2;
-        const R vxy = (G(1, 1, 0) + G(-1, -1, 0) - G(-1, 1, 0) - G(1, -1, 0)) * A.inv_4h2;
-        const R vxz = (G(1, 0, 1) + G(-1, 0, -1) - G(-1, 0, 1) - G(1, 0, -1)) * A.inv_4h2;
-        const R vyz = (G(0, 1, 1) + G(0, -1, -1) - G(0, -1, 1) - G(0, 1, -1)) * A.inv_4h2;
-        const uint32_t ok = k * S;
-        const R f0 = ld_nt(A.fib, ok), f1 = ld_nt(A.fib + A.pitch, ok), f2 = ld_nt(A.fib + 2 * A.pitch, ok);
-        const R b0 = ld_nt(A.drift, ok), b1 = ld_nt(A.drift + A.pitch, ok), b2 = ld_nt(A.drift + 2 * A.pitch, ok);
-        const R dd = A.ddiff, dc = A.dcross;
-        // D = Dc*I + (Df-Dc) f f^T ; du = sum_ij D_ij d_i d_j V + sum_j b_j d_j V
-        du = (dc + dd * f0 * f0) * vxx + (dc + dd * f1 * f1) * vyy + (dc + dd * f2 * f2) * vzz +
-             R(2) * dd * (f0 * f1 * vxy + f0 * f2 * vxz + f1 * f2 * vyz) +
-             (b0 * (vxp - vxm) + b1 * (vyp - vym) + b2 * (vzp - vzm)) * A.inv_2h;
-    } else {
-        const R vzp = V(pb + S), vzm = V(pb - S);
-        du = A.dcross * ((vxm + vxp - R(2) * vc) + (vym + vyp - R(2) * vc) + (vzm + vzp - R(2) * vc)) * A.inv_h2;
-    }
-
-    R ist = R(0);
-    if (A.stim_on) {
-        const float off = A.stim_off[k];
-        if (A.tc >= (double)off && A.tc < (double)off + A.stim_dur) ist = A.stim_strength;
-    }
-    if (A.s2_on && A.s2[k]) ist = A.s2_strength;
+    const R du = diffusion<R, ANISO, FLAT>(A, F, vc, G);
+    const R ist = stimulus(A, k);
 
     CellState<R> s;
-    load_state(s, A.st, k);
+    load_state_chunked(s, A.sr, A.sd, k);
     // cls: bits 0-1 class, bits 2-6 ghost cells mirroring this voxel,
     // bit 7 V held at -100 mV (new_11.cu:1247-1248)
     const int c = A.cls[k];
     const bool held = c & 0x80;
-    R vn = tpord_update<R, false>(s, vc, c & 3, *A.consts, A.dt, ist, du, nullptr);
+    const Consts<R> K = *A.consts;  // uniform: scalar loads
+    R vn = tpord_update<R, false>(s, vc, c & 3, K, A.dt, ist, du, nullptr);
     if (held) vn = R(-100);
-    store_state(s, A.st, k);
+    store_state_chunked(s, A.sr, A.sd, k);
     *(R*)((char*)A.vnew + pb) = vn;
-
-    // the ghost mirrors of this voxel (GPU3D.cu:1566-1604) take the new V
-    // too, so the next step reads a complete grid without a refresh launch
-    const int gc = (c >> 2) & 31;
-    if (__ballot(gc != 0)) {
-        const int lane = threadIdx.x & 63;
-        int x = gc;  // inclusive prefix sum over the wavefront; lanes past
-                     // the last voxel have returned, and they are the top lanes
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
-        const int first = A.gwave[k >> 6] + x - gc;
-        for (int i = 0; i < gc; ++i) *(R*)((char*)A.vnew + (uint32_t)A.gout[first + i] * S) = vn;
-    }
-
-    // V guard of GPU3D.cu:1796-1829, one atomic per wavefront.
-    const bool lo = vn < R(-90) && !held, hi = vn > R(200), bad = !(vn == vn) || !(vn - vn == R(0));
-    const unsigned long long mlo = __ballot(lo), mhi = __ballot(hi), mbad = __ballot(bad);
-    if ((mlo | mhi | mbad) && (threadIdx.x & 63) == (__ffsll((long long)(mlo | mhi | mbad)) - 1)) {
-        if (mlo) atomicAdd(&A.guard[0], (unsigned long long)__popcll(mlo));
-        if (mhi) atomicAdd(&A.guard[1], (unsigned long long)__popcll(mhi));
-        if (mbad) atomicAdd(&A.guard[2], (unsigned long long)__popcll(mbad));
-    }
+    // lanes past the last voxel have returned (they are the top lanes of the
+    // last wavefront), so every lane here holds a voxel
+    write_ghosts(A, k >> 6, (c >> 2) & 31, vn);
+    guard_v(A, vn, held, true);
 }
 
 template <class R>

@@ -85,6 +85,40 @@ __device__ __forceinline__ void store_state(const CellState<R>& s, const SoA<R>&
 #undef LQT_ST
 }
 
+// Chunk-major storage (the tissue's state and static voxel data): voxel k
+// belongs to chunk k / 64, the voxels of one wavefront, and a chunk keeps
+// its slots one after another, 64 values each.  A wavefront then reads and
+// writes its whole state as one contiguous block, and every slot is a
+// constant offset from one per-lane address.
+constexpr int kChunk = 64;
+__host__ __device__ inline int64_t chunk_slot(int64_t k, int j, int nslot) {
+    return ((k >> 6) * nslot + j) * kChunk + (k & 63);
+}
+
+template <class R>
+__device__ __forceinline__ void load_state_chunked(CellState<R>& s, const R* r, const double* d, uint32_t k) {
+    const R* rb = r + (size_t)(k >> 6) * (NR * kChunk) + (k & 63);
+    const double* db = d + (size_t)(k >> 6) * (ND * kChunk) + (k & 63);
+#define LQT_LD(n) s.n = ld_nt(rb, (uint32_t)(RS_##n * kChunk * sizeof(R)));
+    LQT_RFIELDS(LQT_LD)
+#undef LQT_LD
+#define LQT_LD(n) s.n = ld_nt(db, (uint32_t)(DS_##n * kChunk * 8));
+    LQT_DFIELDS(LQT_LD)
+#undef LQT_LD
+}
+
+template <class R>
+__device__ __forceinline__ void store_state_chunked(const CellState<R>& s, R* r, double* d, uint32_t k) {
+    R* rb = r + (size_t)(k >> 6) * (NR * kChunk) + (k & 63);
+    double* db = d + (size_t)(k >> 6) * (ND * kChunk) + (k & 63);
+#define LQT_ST(n) st_nt(s.n, rb, (uint32_t)(RS_##n * kChunk * sizeof(R)));
+    LQT_RFIELDS(LQT_ST)
+#undef LQT_ST
+#define LQT_ST(n) st_nt(s.n, db, (uint32_t)(DS_##n * kChunk * 8));
+    LQT_DFIELDS(LQT_ST)
+#undef LQT_ST
+}
+
 // Location of each of the reference's 40 states (TPORd.cc:200-252 order)
 // in storage: kind 0 = V, 1 = R slot, 2 = D slot, 3 = constant 1 (the dead
 // hsp/jp gates, never updated by TPORd.cc).

@@ -36,6 +36,13 @@ struct Consts {
     int clamp;       // CLAMP: V held (TPORd.cc:929-933)
 };
 
+// Per-class entry by selects, not by an indexed load: with the constants in
+// registers this stays in registers for every lane's class.
+template <class R>
+__host__ __device__ __forceinline__ R pick_class(const R (&a)[3], int cls) {
+    return cls == CLS_EPI ? a[CLS_EPI] : (cls == CLS_M ? a[CLS_M] : a[CLS_ENDO]);
+}
+
 // Stored state of one cell, apart from V (kept by the caller) and the two
 // dead ORd INa gates hsp/jp, which TPORd never updates.
 template <class R>
@@ -315,7 +322,7 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R PhiCaL = R(2.0 * k::F) * r2 * (cass * e2 - R(0.341 * k::cao));
         const R PhiCaNa = R(k::F) * r1 * (R(0.75) * nass * e1 - R(0.75 * k::nao));
         const R PhiCaK = R(k::F) * r1 * (R(0.75) * kss * e1 - R(0.75 * k::ko));
-        const R PCa = c.pca[cls];
+        const R PCa = pick_class(c.pca, cls);
         const R gate_np = s.d * (f * (R(1) - nca) + jca * fca * nca);
         const R gate_p = s.d * (fp * (R(1) - nca) + jca * fcap * nca);
         // PCap = 1.1 PCa; PCaNa = 0.00125 PCa; PCaK = 3.574e-4 PCa (:638-642)
@@ -352,7 +359,7 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R rate_xs2 = R(0.01) * M::exl(v, 1.0 / 20.0, -50.0 / 20.0) + R(0.0193) * M::exl(v, -1.0 / 31.0, -66.54 / 31.0);
         s.xs2 = xs1ss - (xs1ss - s.xs2) * M::ex(-dt * rate_xs2);
         const R KsCa = R(1) + R(0.6) * M::rcp(R(1) + M::pow14(R(3.8e-5) * M::rcp(cai)));
-        IKs = c.gks[cls] * KsCa * s.xs1 * s.xs2 * (v - EKs);
+        IKs = pick_class(c.gks, cls) * KsCa * s.xs1 * s.xs2 * (v - EKs);
     }
 
     // ---- IK1 (:738-751)

@@ -162,7 +162,9 @@ def test_launch_shapes_agree_and_autotune(lqt, prec):
     sfib[..., 0] = np.cos(zz / 40.0)
     sfib[..., 2] = np.sin(zz / 40.0)
     ssites = np.flatnonzero((slab != 255) & (zz < 4))
-    for geo in ((lab, fib, sites), (slab, sfib, ssites)):
+    # a shell of about 366k voxels (5,700 chunks of 64 voxels)
+    big = _blob(4, (112, 104, 96))
+    for geo in ((lab, fib, sites), (slab, sfib, ssites), big):
         ref = None
         for i in range(len(shapes)):
             T = lqt.Tissue(geo[0], geo[1], 0.2, 0.154, 0.0385, lqt.preset("cpu_default"), P, 0.02)
