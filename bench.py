@@ -207,10 +207,12 @@ def main():
     ap.add_argument("--no-strands", action="store_true")
     ap.add_argument("--strand-beats", type=int, default=10, help="beats per strand of the 512-strand sweep")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--profile", choices=["fp32", "fp64", "copy", "cells", "strands"], default=None,
+    ap.add_argument("--profile", choices=["fp32", "fp64", "copy", "cells", "strands", "strand-latency"],
+                    default=None,
                     help="short run of one kernel for rocprofv3 (no JSON line); cells = the ionic update "
                          "alone, --steps updates per launch with the state in registers; strands = the "
-                         "512-strand sweep of --strand-beats beats")
+                         "512-strand sweep of --strand-beats beats; strand-latency = time per step of one "
+                         "strand alone on the GPU by length (one beat at BCL 1000 ms)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (one GPU per box, "
@@ -235,6 +237,19 @@ def main():
     if args.profile == "strands":
         r = strands_throughput(A, 512, args.strand_beats, args.seed)
         log(f"strands: {r}")
+        return
+    if args.profile == "strand-latency":
+        # one strand alone: its workgroup is the only one on its CU, so the
+        # step time is the latency of one cell update at 1 or 2 wavefronts
+        # per SIMD (the critical path of a sweep)
+        p = A.preset("cpu_default", iks_scale=0.5, iso=1, suprep=1, dose_fmol=100.0)
+        for n in (64, 128, 256, 320, 400, 512):
+            st = A.Strands([A.StrandSpec(A.transmural(n, 0.3, 0.3), p, bcl=1000.0, nbeats=1)], A.FP32, 0.02)
+            st.run()
+            st.run()
+            us = st.last_ms() * 1e3 / (1000.0 / 0.02)
+            st.close()
+            log(f"strand-latency: {n} cells, {(n + 63) // 64} wavefronts: {us:.2f} us per step")
         return
     if args.profile == "cells":
         # the ionic update alone on the ventricle's voxel count and type mix

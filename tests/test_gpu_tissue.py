@@ -78,6 +78,53 @@ def test_tissue_fp64_matches_oracle(lqt, oracle, aniso):
         assert rel < 1e-8, (k, rel)
 
 
+def test_written_states_at_chunk_edges_step_like_the_oracle(lqt, oracle):
+    """States written from the host at the first and last lanes of the
+    64-voxel chunks (and in the partial last chunk) are the ones the step
+    kernel reads and writes back: FP64 against the oracle stepping the same
+    written states."""
+    O = oracle
+    lab, fib, sites = _blob(5, (18, 16, 14))
+    dx, dt, nsteps = 0.2, 0.02, 60
+    T = lqt.Tissue(lab, fib, dx, 0.154, 0.0385, lqt.preset("cpu_default"), lqt.FP64, dt)
+    nvox = T.nvox
+    assert nvox % 64 != 0 and nvox > 200
+    picks = [0, 1, 62, 63, 64, 127, 128, nvox - 64, nvox - 2, nvox - 1]
+    rng = np.random.default_rng(7)
+    s0 = O.cell_init(-87.5)
+    written = {}
+    for k in picks:
+        st = s0.copy()
+        st[0] = -87.5 + 140.0 * rng.random()               # V, some above threshold
+        st[9:12] = rng.random(3)                          # INa gates
+        st[1] = 7.0 + rng.random()                        # nai
+        st[5] = 1e-4 * (1.0 + rng.random())               # cai
+        T.write_state(k, st)
+        written[k] = st
+    for k in picks:
+        assert np.array_equal(T.state(k), written[k]), k
+    # the oracle from the same states
+    mask = (lab != 255).astype(np.uint8)
+    g, nv = O.ghost_map(mask)
+    assert nv == nvox
+    cells = np.flatnonzero(mask.ravel())
+    dten, drift = O.tissue_tensor(g, nvox, fib.reshape(-1, 3)[cells].astype(np.float32), 0.154, 0.0385, dx)
+    ct = lab.ravel()[cells].astype(np.uint8)
+    s = np.tile(s0, (nvox, 1))
+    for k, st in written.items():
+        s[k] = st
+    p = O.params()
+    for _ in range(nsteps):
+        O.tissue_step(g, nvox, dten, drift, dx, dt, ct, p, s, np.zeros(nvox))
+    T.step(nsteps)
+    err = np.max(np.abs(T.v() - s[:, 0]))
+    assert err < 1e-7, err
+    for k in picks + [2, 65, nvox - 3]:
+        st = T.state(k)
+        rel = np.max(np.abs(st - s[k]) / np.maximum(np.abs(s[k]), 1e-6))
+        assert rel < 1e-8, (k, rel)
+
+
 def test_tissue_fp32_activation_within_one_step(lqt, oracle):
     O = oracle
     lab, fib, sites = _blob(1)
