@@ -75,6 +75,10 @@ struct Math<float> {
     static __device__ __forceinline__ float exl(float v, double s, double o) {
         return __builtin_amdgcn_exp2f(fmaf(v, (float)(s * 1.4426950408889634), (float)(o * 1.4426950408889634)));
     }
+    // Rush-Larsen decay exp(-dt * rate) as exp2(rate * ndt) with
+    // ndt = -dt * log2(e) formed once per update.
+    static __device__ __forceinline__ float decay_scale(float dt) { return -dt * 1.4426950408889634f; }
+    static __device__ __forceinline__ float decay(float ndt, float rate) { return __builtin_amdgcn_exp2f(rate * ndt); }
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float log(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }
@@ -95,6 +99,9 @@ struct Math<double> {
     static __device__ __forceinline__ double ex(double x) { return ::exp(x); }
     template <int DUMMY = 0>
     static __device__ __forceinline__ double exl(double v, double s, double o) { return ::exp(v * s + o); }
+    // exp(-dt * rate): (-dt) * rate rounds exactly as -(dt * rate)
+    static __device__ __forceinline__ double decay_scale(double dt) { return -dt; }
+    static __device__ __forceinline__ double decay(double ndt, double rate) { return ::exp(rate * ndt); }
     static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
     static __device__ __forceinline__ double div(double a, double b) { return a / b; }
     static __device__ __forceinline__ double log(double x) { return ::log(x); }
@@ -199,6 +206,7 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
     const R CaMKa = CaMKb + CaMKt;
     const R fCaMKp = M::rcp(R(1) + R(k::KmCaMK) * M::rcp(CaMKa));  // fINaLp = fItop = fICaLp = fJrelp = fJupp
     const R vfrt = v * R(k::FRT);
+    const R ndt = M::decay_scale(dt);
 
     // ---- TP06 INa (:465-504)
     {
@@ -223,9 +231,9 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         }
         const R eh = R(1) + M::exl(v, 1.0 / 7.43, 71.55 / 7.43);
         const R H_INF = M::rcp(eh * eh);
-        s.sm = M_INF - (M_INF - s.sm) * M::ex(-dt * M::rcp(TAU_M));
-        s.sh = H_INF - (H_INF - s.sh) * M::ex(-dt * rate_h);
-        s.sj = H_INF - (H_INF - s.sj) * M::ex(-dt * rate_j);
+        s.sm = M_INF - (M_INF - s.sm) * M::decay(ndt, M::rcp(TAU_M));
+        s.sh = H_INF - (H_INF - s.sh) * M::decay(ndt, rate_h);
+        s.sj = H_INF - (H_INF - s.sj) * M::decay(ndt, rate_j);
     }
     const R INa = R(14.838) * s.sm * s.sm * s.sm * s.sh * s.sj * (v - ENa);
 
@@ -233,11 +241,11 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
     {
         const R mLss = M::rcp(R(1) + M::exl(v, -1.0 / 5.264, -42.85 / 5.264));
         const R tm_rate = R(6.765) * M::exl(v, 1.0 / 34.77, 11.64 / 34.77) + R(8.552) * M::exl(v, -1.0 / 5.955, -77.42 / 5.955);
-        s.mL = mLss - (mLss - s.mL) * M::ex(-dt * tm_rate);
+        s.mL = mLss - (mLss - s.mL) * M::decay(ndt, tm_rate);
         const R hLss = M::rcp(R(1) + M::exl(v, 1.0 / 7.488, 87.61 / 7.488));
-        s.hL = hLss - (hLss - s.hL) * M::ex(-dt * R(1.0 / 200.0));
+        s.hL = hLss - (hLss - s.hL) * M::decay(ndt, R(1.0 / 200.0));
         const R hLssp = M::rcp(R(1) + M::exl(v, 1.0 / 7.488, 93.81 / 7.488));
-        s.hLp = hLssp - (hLssp - s.hLp) * M::ex(-dt * R(1.0 / 600.0));
+        s.hLp = hLssp - (hLssp - s.hLp) * M::decay(ndt, R(1.0 / 600.0));
     }
     const R GNaL = epi ? R(0.0075 * 0.6) : R(0.0075);
     const R INaL = GNaL * (v - ENa) * s.mL * ((R(1) - fCaMKp) * s.hL + fCaMKp * s.hLp);
@@ -249,7 +257,7 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R ta = R(1.0515) * M::rcp(M::rcp(R(1.2089) * (R(1) + M::exl(v, -1.0 / 29.3814, 18.4099 / 29.3814))) +
                                         R(3.5) * M::rcp(R(1) + M::exl(v, 1.0 / 29.3814, 100.0 / 29.3814)));
         const R inv_ta = M::rcp(ta);
-        const R e_ta = M::ex(-dt * inv_ta);
+        const R e_ta = M::decay(ndt, inv_ta);
         s.a = ass - (ass - s.a) * e_ta;
         const R iss = M::rcp(R(1) + M::exl(v, 1.0 / 5.711, 43.94 / 5.711));
         R tiF = R(4.562) + M::rcp(R(0.3933) * M::exl(v, -1.0 / 100.0, -1.0) + R(0.08004) * M::exl(v, 1.0 / 16.59, 50.0 / 16.59));
@@ -262,8 +270,8 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         }
         const R AiF = M::rcp(R(1) + M::exl(v, 1.0 / 151.2, -213.6 / 151.2));
         const R AiS = R(1) - AiF;
-        s.iF = iss - (iss - s.iF) * M::ex(-dt * M::rcp(tiF));
-        s.iS = iss - (iss - s.iS) * M::ex(-dt * M::rcp(tiS));
+        s.iF = iss - (iss - s.iF) * M::decay(ndt, M::rcp(tiF));
+        s.iS = iss - (iss - s.iS) * M::decay(ndt, M::rcp(tiS));
         const R i_ = AiF * s.iF + AiS * s.iS;
         const R assp = M::rcp(R(1) + M::exl(v, -1.0 / 14.82, 24.34 / 14.82));
         s.ap = assp - (assp - s.ap) * e_ta;
@@ -271,8 +279,8 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
                                                              M::exl(v, -1.0 / 0.2154, 12.23 / 0.2154));
         const R dti_recover = R(1) - R(0.5) * M::rcp(R(1) + M::exl(v, 1.0 / 20.0, 70.0 / 20.0));
         const R dd = dti_develop * dti_recover;
-        s.iFp = iss - (iss - s.iFp) * M::ex(-dt * M::rcp(dd * tiF));
-        s.iSp = iss - (iss - s.iSp) * M::ex(-dt * M::rcp(dd * tiS));
+        s.iFp = iss - (iss - s.iFp) * M::decay(ndt, M::rcp(dd * tiF));
+        s.iSp = iss - (iss - s.iSp) * M::decay(ndt, M::rcp(dd * tiS));
         const R ip = AiF * s.iFp + AiS * s.iSp;
         const R Gto = (epi || mid) ? R(0.08) : R(0.02);
         Ito = Gto * (v - EK) * ((R(1) - fCaMKp) * s.a * i_ + fCaMKp * s.ap * ip);
@@ -288,14 +296,14 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R vs = v + c.dss_shift;
         const R dss = M::rcp(R(1) + M::exl(vs, -1.0 / 4.230, -3.940 / 4.230));
         const R td = R(0.6) + M::rcp(M::exl(v, -0.05, -0.3) + M::exl(v, 0.09, 1.26));
-        s.d = dss - (dss - s.d) * M::ex(-dt * M::rcp(td));
+        s.d = dss - (dss - s.d) * M::decay(ndt, M::rcp(td));
         const R fss = M::rcp(R(1) + M::exl(v + c.fss_shift, 1.0 / 3.696, 19.58 / 3.696));
         const R tff = R(7) + M::rcp(R(0.0045) * M::exl(v, -0.1, -2.0) + R(0.0045) * M::exl(v, 0.1, 2.0));
         const R tfs = R(1000) + M::rcp(R(0.000035) * M::exl(v, -0.25, -1.25) + R(0.000035) * M::exl(v, 1.0 / 6.0, 5.0 / 6.0));
         const R Aff = R(0.6), Afs = R(0.4);
         const R inv_tff = M::rcp(tff);
-        s.ff = fss - (fss - s.ff) * M::ex(-dt * inv_tff);
-        s.fs = fss - (fss - s.fs) * M::ex(-dt * M::rcp(tfs));
+        s.ff = fss - (fss - s.ff) * M::decay(ndt, inv_tff);
+        s.fs = fss - (fss - s.fs) * M::decay(ndt, M::rcp(tfs));
         const R f = Aff * s.ff + Afs * s.fs;
         const R fcass = fss;
         const R tfcaf = R(7) + M::rcp(R(0.04) * M::exl(v, -1.0 / 7.0, 4.0 / 7.0) + R(0.04) * M::exl(v, 1.0 / 7.0, -4.0 / 7.0));
@@ -303,13 +311,13 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R Afcaf = R(0.3) + R(0.6) * M::rcp(R(1) + M::exl(v, 0.1, -1.0));
         const R Afcas = R(1) - Afcaf;
         const R inv_tfcaf = M::rcp(tfcaf);
-        s.fcaf = fcass - (fcass - s.fcaf) * M::ex(-dt * inv_tfcaf);
-        s.fcas = fcass - (fcass - s.fcas) * M::ex(-dt * M::rcp(tfcas));
+        s.fcaf = fcass - (fcass - s.fcaf) * M::decay(ndt, inv_tfcaf);
+        s.fcas = fcass - (fcass - s.fcas) * M::decay(ndt, M::rcp(tfcas));
         const R fca = Afcaf * s.fcaf + Afcas * s.fcas;
-        s.jca = fcass - (fcass - s.jca) * M::ex(-dt * R(1.0 / 75.0));
-        s.ffp = fss - (fss - s.ffp) * M::ex(-dt * R(1.0 / 2.5) * inv_tff);
+        s.jca = fcass - (fcass - s.jca) * M::decay(ndt, R(1.0 / 75.0));
+        s.ffp = fss - (fss - s.ffp) * M::decay(ndt * R(1.0 / 2.5), inv_tff);
         const R fp = Aff * s.ffp + Afs * s.fs;
-        s.fcafp = fcass - (fcass - s.fcafp) * M::ex(-dt * R(1.0 / 2.5) * inv_tfcaf);
+        s.fcafp = fcass - (fcass - s.fcafp) * M::decay(ndt * R(1.0 / 2.5), inv_tfcaf);
         const R fcap = Afcaf * s.fcafp + Afcas * s.fcas;
         // nca (:613-617): anca*k2n/km2n relaxes with rate km2n = jca
         const R km2n = s.jca;
@@ -317,7 +325,7 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R q2 = q * q;
         const R anca = M::rcp(R(1000) * M::rcp(km2n) + q2 * q2);
         const R ninf = anca * R(1000) * M::rcp(km2n);
-        s.nca = ninf - (ninf - s.nca) * M::ex(-km2n * dt);
+        s.nca = ninf - (ninf - s.nca) * M::decay(ndt, km2n);
         const R nca = s.nca, jca = s.jca;
         const R PhiCaL = R(2.0 * k::F) * r2 * (cass * e2 - R(0.341 * k::cao));
         const R PhiCaNa = R(k::F) * r1 * (R(0.75) * nass * e1 - R(0.75 * k::nao));
@@ -341,8 +349,8 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R txrs = R(1.865) + M::rcp(R(0.06629) * M::exl(v, 1.0 / 7.355, -34.70 / 7.355) +
                                          R(1.128e-5) * M::exl(v, -1.0 / 25.94, 29.74 / 25.94));
         const R Axrf = M::rcp(R(1) + M::exl(v, 1.0 / 38.21, 54.81 / 38.21));
-        s.xrf = xrss - (xrss - s.xrf) * M::ex(-dt * M::rcp(txrf));
-        s.xrs = xrss - (xrss - s.xrs) * M::ex(-dt * M::rcp(txrs));
+        s.xrf = xrss - (xrss - s.xrf) * M::decay(ndt, M::rcp(txrf));
+        s.xrs = xrss - (xrss - s.xrs) * M::decay(ndt, M::rcp(txrs));
         const R xr = Axrf * s.xrf + (R(1) - Axrf) * s.xrs;
         const R rkr = M::rcp((R(1) + M::exl(v, 1.0 / 75.0, 55.0 / 75.0)) * (R(1) + M::exl(v, 1.0 / 30.0, -10.0 / 30.0)));
         const R GKr = epi ? R(0.046 * 1.3) : (mid ? R(0.046 * 0.8) : R(0.046));
@@ -355,9 +363,9 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R xs1ss = M::rcp(R(1) + M::exl(v, -1.0 / 8.932, -11.60 / 8.932));
         const R txs1 = R(817.3) + M::rcp(R(2.326e-4) * M::exl(v, 1.0 / 17.80, 48.28 / 17.80) +
                                          R(0.001292) * M::exl(v, -1.0 / 230.0, -210.0 / 230.0));
-        s.xs1 = xs1ss - (xs1ss - s.xs1) * M::ex(-dt * M::rcp(txs1));
+        s.xs1 = xs1ss - (xs1ss - s.xs1) * M::decay(ndt, M::rcp(txs1));
         const R rate_xs2 = R(0.01) * M::exl(v, 1.0 / 20.0, -50.0 / 20.0) + R(0.0193) * M::exl(v, -1.0 / 31.0, -66.54 / 31.0);
-        s.xs2 = xs1ss - (xs1ss - s.xs2) * M::ex(-dt * rate_xs2);
+        s.xs2 = xs1ss - (xs1ss - s.xs2) * M::decay(ndt, rate_xs2);
         const R KsCa = R(1) + R(0.6) * M::rcp(R(1) + M::pow14(R(3.8e-5) * M::rcp(cai)));
         IKs = pick_class(c.gks, cls) * KsCa * s.xs1 * s.xs2 * (v - EKs);
     }
@@ -367,7 +375,7 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
     {
         const R xk1ss = M::rcp(R(1) + M::exl(v, -1.0 / (1.5692 * k::ko + 3.8115), -(2.5538 * k::ko + 144.59) / (1.5692 * k::ko + 3.8115)));
         const R rate_k1 = (M::exl(v, -1.0 / 20.36, -127.2 / 20.36) + M::exl(v, 1.0 / 69.33, 236.8 / 69.33)) * R(1.0 / 122.2);
-        s.xk1 = xk1ss - (xk1ss - s.xk1) * M::ex(-dt * rate_k1);
+        s.xk1 = xk1ss - (xk1ss - s.xk1) * M::decay(ndt, rate_k1);
         const R rk1 = M::rcp(R(1) + M::exl(v, 1.0 / 9.493, (105.8 - 2.6 * k::ko) / 9.493));
         const R GK1 = epi ? R(0.1908 * 1.2) : (mid ? R(0.1908 * 1.3) : R(0.1908));
         IK1 = GK1 * M::sqrtc(k::ko) * rk1 * s.xk1 * (v - EK);
@@ -463,11 +471,11 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
     const R q_rel = M::rcp(R(1) + R(0.0123) * M::rcp(cajsr));
     R tau_rel = R(4.75) * q_rel;
     tau_rel = tau_rel < R(0.005) ? R(0.005) : tau_rel;
-    s.Jrelnp = Jrel_inf - (Jrel_inf - s.Jrelnp) * M::ex(-dt * M::rcp(tau_rel));
+    s.Jrelnp = Jrel_inf - (Jrel_inf - s.Jrelnp) * M::decay(ndt, M::rcp(tau_rel));
     const R Jrel_infp = R(0.5 * 1.25 * 4.75) * (-ICaL) * jrel_den * mfac;
     R tau_relp = R(1.25 * 4.75) * q_rel;
     tau_relp = tau_relp < R(0.005) ? R(0.005) : tau_relp;
-    s.Jrelp = Jrel_infp - (Jrel_infp - s.Jrelp) * M::ex(-dt * M::rcp(tau_relp));
+    s.Jrelp = Jrel_infp - (Jrel_infp - s.Jrelp) * M::decay(ndt, M::rcp(tau_relp));
     const R Jrel = (R(1) - fCaMKp) * s.Jrelnp + fCaMKp * s.Jrelp;
 
     const R upf = epi ? R(1.3) : R(1);
