@@ -124,11 +124,14 @@ typedef struct {
 
 /* S1 train of GPU3D.cu:1757-1762 / set_Istim :397-409: site k receives
  * `strength` while the in-cycle time t - floor(t/bcl)*bcl lies in
- * [offset_k, offset_k + duration); optional S2 of set_S2Istim :411-422:
- * `s2_strength` on the S2 sites while t in [s2_time, s2_time + s2_duration). */
+ * [offset_k, offset_k + duration) and, if window > 0, below window
+ * (GPU3D.cu:1757 only calls set_Istim while the in-cycle time is below
+ * 82 ms); optional S2 of set_S2Istim :411-422: `s2_strength` on the S2
+ * sites while t in [s2_time, s2_time + s2_duration). */
 typedef struct {
     double bcl, duration, strength;
     double s2_time, s2_duration, s2_strength;  /* s2_duration 0 = no S2 */
+    double window;                             /* ms; 0 = no limit */
 } lqt_stim_protocol;
 
 typedef struct lqt_tissue lqt_tissue;

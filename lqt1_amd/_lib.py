@@ -52,7 +52,8 @@ class Geometry(C.Structure):
 
 class StimProtocol(C.Structure):
     _fields_ = [("bcl", C.c_double), ("duration", C.c_double), ("strength", C.c_double),
-                ("s2_time", C.c_double), ("s2_duration", C.c_double), ("s2_strength", C.c_double)]
+                ("s2_time", C.c_double), ("s2_duration", C.c_double), ("s2_strength", C.c_double),
+                ("window", C.c_double)]
 
 
 class StrandDesc(C.Structure):

@@ -455,7 +455,8 @@ lqt_status lqt_tissue_info(const lqt_tissue* t, int64_t* nvox, int64_t* nghost) 
 lqt_status lqt_tissue_set_stimulus(lqt_tissue* t, const lqt_stim_protocol* p, int64_t nsites,
                                    const int64_t* site_cells, const float* site_offset_ms, int64_t ns2,
                                    const int64_t* s2_cells) {
-    LQT_CHECK(t && p && p->bcl > 0 && p->duration >= 0, LQT_ERR_ARG, "lqt_tissue_set_stimulus: bad protocol");
+    LQT_CHECK(t && p && p->bcl > 0 && p->duration >= 0 && p->window >= 0, LQT_ERR_ARG,
+              "lqt_tissue_set_stimulus: bad protocol");
     LQT_CHECK(nsites >= 0 && ns2 >= 0 && (nsites == 0 || site_cells) && (ns2 == 0 || s2_cells), LQT_ERR_ARG,
               "lqt_tissue_set_stimulus: bad site arrays");
     const int64_t ncell = (int64_t)t->nx * t->ny * t->nz;
@@ -531,7 +532,8 @@ lqt_status lqt_tissue_step_async(lqt_tissue* t, int64_t nsteps, void* stream) {
         const int64_t ti = i / 16;
         const bool timed = (i % 16) == 0 && ti < ntime;
         const double tc = t->time - floor(t->time / t->proto.bcl) * t->proto.bcl;
-        const int stim_on = t->nsites > 0 && tc >= t->off_min && tc < t->off_max + t->proto.duration;
+        const int stim_on = t->nsites > 0 && tc >= t->off_min && tc < t->off_max + t->proto.duration &&
+                            (t->proto.window <= 0 || tc < t->proto.window);
         const int s2_on = t->ns2 > 0 && t->proto.s2_duration > 0 && t->time >= t->proto.s2_time &&
                           t->time < t->proto.s2_time + t->proto.s2_duration;
         if (timed) LQT_HIP(hipEventRecord(t->ev[2 * ti], s));

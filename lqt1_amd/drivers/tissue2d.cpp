@@ -156,7 +156,8 @@ int main(int argc, char** argv) {
         fclose(fs);
     }
     printf("There are %lld stimulus points.\n\n", (long long)sites.size());
-    lqt_stim_protocol sp = {bcl, 3.0, -52.0, s2_time, s2_time >= 0 ? 3.0 : 0.0, -104.0};
+    // set_Istim only while the in-cycle time is below 82 ms (new_11.cu:1785)
+    lqt_stim_protocol sp = {bcl, 3.0, -52.0, s2_time, s2_time >= 0 ? 3.0 : 0.0, -104.0, 82.0};
     std::vector<float> off(sites.size(), 0.0f);
     if (lqt_tissue_set_stimulus(t, &sp, (int64_t)sites.size(), sites.data(), off.data(), (int64_t)s2cells.size(),
                                 s2cells.data()))

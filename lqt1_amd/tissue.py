@@ -32,12 +32,13 @@ class Tissue:
         self.nvox, self.nghost = nv.value, ng.value
 
     def set_stimulus(self, sites, offsets_ms=None, bcl=1000.0, duration=1.0, strength=-52.0,
-                     s2_sites=None, s2_time=0.0, s2_duration=0.0, s2_strength=-104.0):
-        """sites: grid indices (x*ny + y)*nz + z; offsets_ms: per-site start within the cycle."""
+                     s2_sites=None, s2_time=0.0, s2_duration=0.0, s2_strength=-104.0, window=0.0):
+        """sites: grid indices (x*ny + y)*nz + z; offsets_ms: per-site start within the cycle;
+        window: S1 only while the in-cycle time is below it (0 = no limit)."""
         sites = np.ascontiguousarray(sites, np.int64)
         offs = None if offsets_ms is None else np.ascontiguousarray(offsets_ms, np.float32)
         s2 = None if s2_sites is None else np.ascontiguousarray(s2_sites, np.int64)
-        p = L.StimProtocol(bcl, duration, strength, s2_time, s2_duration, s2_strength)
+        p = L.StimProtocol(bcl, duration, strength, s2_time, s2_duration, s2_strength, window)
         L.check(L.lib().lqt_tissue_set_stimulus(self.h, C.byref(p), sites.size, L.ptr(sites), L.ptr(offs),
                                                 0 if s2 is None else s2.size, L.ptr(s2)))
 
