@@ -53,6 +53,8 @@ class Strands:
     def __init__(self, specs, precision=L.FP32, dt=0.02):
         self.specs = list(specs)
         n = len(self.specs)
+        if n == 0:  # the library refuses an empty batch; let it say so
+            L.check(L.lib().lqt_strands_create(0, None, None, None, None, 0, precision, dt, C.byref(C.c_void_p())))
         descs = (L.StrandDesc * n)()
         cts, coeffs, plist = [], [], []
         off = 0

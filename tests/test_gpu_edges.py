@@ -1,0 +1,93 @@
+"""Edge cases of the three engines on the GPU.
+
+* A tissue of one voxel (every stencil neighbour is a ghost mirroring it, so
+  the diffusion term is exactly zero) steps exactly as one cell of the cell
+  engine with the same stimulus: identical bits of all 40 states, FP64 and
+  FP32, isotropic and anisotropic.
+* Strands at the wavefront edges (2, 63, 64, 65 and 128 cells) against the
+  FP64 oracle of OneD.cc, in the per-size and the packed layouts.
+* Empty inputs, and strands outside the documented 2..512 cells, are
+  refused with an error, not run.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _stim_steps(dur, dt, nsteps):
+    """Steps whose time t (accumulated t += dt from 0) lies in [0, dur)."""
+    t, on = 0.0, []
+    for _ in range(nsteps):
+        on.append(0.0 <= t < dur)
+        t += dt
+    return np.array(on)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("aniso", [False, True])
+def test_one_voxel_tissue_is_one_cell(lqt, prec, aniso):
+    P = lqt.FP64 if prec == "fp64" else lqt.FP32
+    lab = np.full((3, 3, 3), 255, np.uint8)
+    lab[1, 1, 1] = lqt.MCELL
+    fib = None
+    if aniso:
+        fib = np.zeros((3, 3, 3, 3), np.float32)
+        fib[1, 1, 1] = (0.6, 0.8, 0.0)
+    p = lqt.preset("cpu_default")
+    T = lqt.Tissue(lab, fib, 0.2, 0.154, 0.0385 if aniso else 0.154, p, P, 0.02)
+    assert T.nvox == 1 and T.nghost == 26
+    T.set_stimulus([13], None, bcl=1000.0, duration=1.0, strength=-52.0)
+    c = lqt.Cells(np.array([lqt.MCELL], np.uint8), p, P, 0.02)
+    nsteps = 1500
+    on = _stim_steps(1.0, 0.02, nsteps)
+    k = int(on.sum())
+    assert on[:k].all() and not on[k:].any()
+    T.step(nsteps)
+    c.step(k, istim=-52.0)
+    c.step(nsteps - k)
+    a, b = T.state(0), c.state()[0]
+    assert a[0] > -90.0 and np.max(np.abs(a - b)) == 0.0, np.flatnonzero(a != b)
+
+
+def _oracle(O, spec):
+    op = O.params(iks_scale=spec.params.iks_scale, iso=spec.params.iso, pca_iso=spec.params.pca_iso,
+                  suprep=spec.params.suprep, dose_fmol=spec.params.dose_fmol)
+    return O.strand_run(spec.celltype, spec.coeff, op, dx=spec.dx, bcl=spec.bcl, stop_time=spec.nbeats * spec.bcl,
+                        stim=spec.stim_strength, stim_dur=spec.stim_duration, stim_start=spec.stim_start,
+                        nstim=spec.nstim, rec_every=0, record=False, act_beat=spec.nbeats - 1)
+
+
+def test_strands_at_wavefront_edges_match_oracle(lqt, oracle):
+    O = oracle
+    lengths = [2, 63, 64, 65, 128]
+    specs = [lqt.StrandSpec(lqt.transmural(n), lqt.preset("cpu_default"), bcl=400.0, nbeats=1, stim_start=5.0,
+                            nstim=min(3, n)) for n in lengths]
+    ref = [_oracle(O, s) for s in specs]
+    for mode in (0, 2):
+        st = lqt.Strands(specs, lqt.FP64, 0.02)
+        st.set_mode(mode)
+        st.run()
+        res = st.results()
+        for i, (n, r) in enumerate(zip(lengths, ref)):
+            act, apd, ead = st.cell_results(i)
+            assert act.size == n and res[i]["steps"] == r["steps"], (mode, n)
+            assert np.all(act > 0), (mode, n, act)
+            assert np.max(np.abs(act - r["act"])) < 1e-6, (mode, n)
+            fin = st.final_state(i)
+            assert np.max(np.abs(fin[:, 0] - r["state"][:, 0])) < 1e-6, (mode, n)
+            rel = np.max(np.abs(fin - r["state"]) / np.maximum(np.abs(r["state"]), 1e-6))
+            assert rel < 1e-6, (mode, n, rel)
+        st.close()
+
+
+def test_empty_inputs_are_refused(lqt):
+    with pytest.raises(lqt.LqtError):
+        lqt.Tissue(np.full((4, 4, 4), 255, np.uint8))
+    with pytest.raises(lqt.LqtError):
+        lqt.Strands([])
+    for n in (1, 513):
+        with pytest.raises(lqt.LqtError):
+            lqt.Strands([lqt.StrandSpec(lqt.transmural(n), lqt.preset("cpu_default"), nbeats=1)])
+    with pytest.raises(lqt.LqtError):
+        lqt.Cells(np.zeros(0, np.uint8))
