@@ -146,6 +146,43 @@ def strand_specs(A, nstr, nbeats, seed=0):
     return specs
 
 
+def sheet_run(A, sim_ms=2000.0, compare_ms=250.0, seed=0):
+    """The 2D annulus sheet (600 x 600 @ 0.15 mm, isotropic D 0.154, about
+    97.7k voxels): S1 from the endo rim every 1000 ms (3 ms), S2 over one
+    quadrant at the seed's coupling interval (3 ms, -104 pA/pF), sim_ms of
+    simulated time in FP32 and in FP64.  Reports cell-steps/s of each and
+    V L-inf between them after the S1 wave (compare_ms)."""
+    import numpy as np
+
+    from lqt1_amd import geometry as G
+
+    geo = G.annulus_2d(seed=seed)
+    ci = float(geo.info["s2_ci"])
+    dt = 0.02
+    n_cmp = int(round(compare_ms / dt))
+    n_all = int(round(sim_ms / dt))
+    out, vcmp = {}, {}
+    for name, prec in (("fp32", A.FP32), ("fp64", A.FP64)):
+        T = A.Tissue(geo.label, None, geo.dx, 0.154, 0.154, A.preset("config1"), prec, dt)
+        T.set_stimulus(geo.stim_sites, None, bcl=1000.0, duration=3.0, strength=-52.0, s2_sites=geo.s2_sites,
+                       s2_time=ci, s2_duration=3.0, s2_strength=-104.0)
+        T.step(n_cmp)
+        vcmp[name] = T.v()
+        A.lib().lqt_device_synchronize()
+        t0 = time.perf_counter()
+        T.step(n_all - n_cmp)
+        wall = time.perf_counter() - t0
+        us_main, _ = T.kernel_us()
+        out[name] = dict(cell_steps_per_s=T.nvox * (n_all - n_cmp) / wall, ms_per_step=wall / (n_all - n_cmp) * 1e3,
+                         kernel_us=round(us_main, 2), guard=T.v_guard().tolist())
+        nvox = T.nvox
+        T.close()
+    return dict(workload=f"2D annulus 600x600 @0.15 mm, isotropic D 0.154, S1 BCL 1000 ms + S2 at {ci:.0f} ms "
+                         f"over a quadrant, {sim_ms:.0f} ms", voxels=nvox, steps=n_all, **out,
+                fp32_over_fp64=out["fp32"]["cell_steps_per_s"] / out["fp64"]["cell_steps_per_s"],
+                v_linf_mv_fp32_vs_fp64_at_ms=[compare_ms, float(np.max(np.abs(vcmp["fp32"] - vcmp["fp64"])))])
+
+
 def strands_cpu(specs, nsample, threads):
     """The FP64 oracle's strand loop (OneD.cc) on the first nsample strands
     of the sweep for one beat each, one strand per host thread; returns
@@ -205,14 +242,16 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--tune-steps", type=int, default=20, help="autotuner trial steps per launch shape (0 = off)")
     ap.add_argument("--no-strands", action="store_true")
+    ap.add_argument("--no-sheet", action="store_true", help="skip the 2D annulus sheet leg")
     ap.add_argument("--strand-beats", type=int, default=10, help="beats per strand of the 512-strand sweep")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--profile", choices=["fp32", "fp64", "copy", "cells", "strands", "strand-latency"],
+    ap.add_argument("--profile", choices=["fp32", "fp64", "copy", "cells", "strands", "strand-latency", "sheet"],
                     default=None,
                     help="short run of one kernel for rocprofv3 (no JSON line); cells = the ionic update "
                          "alone, --steps updates per launch with the state in registers; strands = the "
                          "512-strand sweep of --strand-beats beats; strand-latency = time per step of one "
-                         "strand alone on the GPU by length (one beat at BCL 1000 ms)")
+                         "strand alone on the GPU by length (one beat at BCL 1000 ms); sheet = the 2D annulus "
+                         "leg alone")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (one GPU per box, "
@@ -237,6 +276,9 @@ def main():
     if args.profile == "strands":
         r = strands_throughput(A, 512, args.strand_beats, args.seed)
         log(f"strands: {r}")
+        return
+    if args.profile == "sheet":
+        log(f"sheet2d: {sheet_run(A)}")
         return
     if args.profile == "strand-latency":
         # one strand alone: its workgroup is the only one on its CU, so the
@@ -314,6 +356,14 @@ def main():
         except Exception as e:
             strands = dict(error=str(e))
 
+    sheet = None
+    if not args.no_sheet:
+        try:
+            sheet = sheet_run(A)
+            log(f"sheet2d: {sheet}")
+        except Exception as e:
+            sheet = dict(error=str(e))
+
     cpu = None
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
@@ -350,6 +400,7 @@ def main():
         "fp32_over_fp64": v32 / v64,
         "accuracy": acc,
         "strands": strands,
+        "sheet2d": sheet,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
