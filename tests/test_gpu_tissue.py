@@ -192,8 +192,10 @@ def test_checkpoint_and_vtk(lqt, tmp_path):
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
 def test_launch_shapes_agree_and_autotune(lqt, prec):
     """Every launch shape of the fused step (block size, XCD map, register
-    budget) computes each voxel the same way: identical V and states.  The
-    autotuner leaves the simulation where it was and keeps the fastest."""
+    budget, LDS staging) computes each voxel the same way: identical V and
+    states, in 3D and on a 2D sheet (which runs shape 0 for every shape).
+    The autotuner leaves the simulation where it was and keeps the
+    fastest."""
     P = lqt.FP32 if prec == "fp32" else lqt.FP64
     lab, fib, sites = _blob(3, (16, 14, 12))
     shapes = lqt.Tissue.variants()
@@ -211,13 +213,23 @@ def test_launch_shapes_agree_and_autotune(lqt, prec):
     ssites = np.flatnonzero((slab != 255) & (zz < 4))
     # a shell of about 366k voxels (5,700 chunks of 64 voxels)
     big = _blob(4, (112, 104, 96))
-    for geo in ((lab, fib, sites), (slab, sfib, ssites), big):
+    # a 2D ring sheet with holes and tangential fibres
+    xx, yy = np.meshgrid(np.arange(90), np.arange(80), indexing="ij")
+    rr = np.hypot(xx - 44.5, yy - 39.5)
+    ring = (rr > 20) & (rr < 34) & (np.random.default_rng(2).random(rr.shape) > 0.02)
+    sheet = np.where(ring, np.where(rr < 25, 3, np.where(rr < 29, 4, 2)), 255).astype(np.uint8)[:, :, None]
+    th = np.arctan2(yy - 39.5, xx - 44.5)
+    shf = np.zeros((90, 80, 1, 3), np.float32)
+    shf[:, :, 0, 0], shf[:, :, 0, 1] = -np.sin(th), np.cos(th)
+    shsites = np.flatnonzero(ring & (rr < 22))
+    for geo in ((lab, fib, sites), (slab, sfib, ssites), big, (sheet, shf, shsites)):
         ref = None
         for i in range(len(shapes)):
             T = lqt.Tissue(geo[0], geo[1], 0.2, 0.154, 0.0385, lqt.preset("cpu_default"), P, 0.02)
             T.set_stimulus(geo[2], None, strength=-80.0)
             T.set_variant(i)
-            T.step(120)
+            T.step(57)
+            T.step(263)
             out = (T.v(), T.state(T.nvox // 3))
             if ref is None:
                 ref = out
