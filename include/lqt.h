@@ -349,6 +349,44 @@ lqt_status lqt_mlp_grads(lqt_mlp *m, int n, const float *x, const float *y, cons
 lqt_status lqt_mlp_train(lqt_mlp *m, int64_t n, const float *x, const float *y, const float *mask, int epochs,
                          int batch, double lr, uint64_t seed, float *loss_out);
 
+/* Surrogate-steered sweep over an IKs-scale x dose x ISO grid of one strand
+ * shape (no counterpart in the reference).  The network (trained on inputs
+ * [2 iks - 1, 2 iso - 1, 2 dose - 1, (bcl - 1000)/500, (ncell - 225)/175,
+ * (mfrac - 0.35)/0.15] and outputs [(QT - 350)/100, APD90 x3 likewise, EAD
+ * logit]) predicts every point; each point whose predicted QT is within
+ * band_ms of qt_threshold or whose EAD probability lies in (p_lo, p_hi) is
+ * re-run as an OneD.cc strand (S1 -52 pA/pF, 1 ms, cells 0-2, from 50 ms;
+ * dx 0.15 mm, D 0.154; ENDO (1 - mfrac)/2, then MCELL mfrac, then EPI;
+ * SUPREP at dose * dose_max_fmol) by the batched solver, whose QT and EAD
+ * replace the prediction. */
+typedef struct {
+    double bcl;            /* ms                                          */
+    int ncell;             /* strand length, 2..512                       */
+    double mfrac;          /* M-cell fraction                             */
+    int nbeats;            /* QT, APD90 and EAD of the last beat          */
+    double qt_threshold;   /* ms: long QT when QT > qt_threshold          */
+    double band_ms;
+    double p_lo, p_hi;
+    double dose_max_fmol;  /* dose 1 = this many fmol                     */
+    int chunk;             /* strands per solver launch; 0 = 4096         */
+    int autotune_steps;    /* launch-layout trial steps per chunk; 0 = 2000 */
+} lqt_sweep_desc;
+
+typedef struct {
+    int64_t points, rechecked;
+    double ead_agree;      /* re-checked points whose predicted EAD call was the solver's */
+    double t_surrogate_s;  /* inputs, network and decoding                */
+    double t_recheck_s;    /* building, tuning and running the re-check strands */
+} lqt_sweep_stats;
+
+/* Grid point (a, b, c) = (iks[a], dose[b], iso[c]) at index (a*n_dose + b)*n_iso + c.
+ * Per point: qt (ms), ead and long_qt decisions, rechecked flag, and
+ * qt_err = predicted - solver QT on re-checked points (NaN elsewhere). */
+lqt_status lqt_surrogate_sweep(lqt_mlp *net, int n_iks, const double *iks, int n_dose, const double *dose,
+                               int n_iso, const int *iso, const lqt_sweep_desc *desc, lqt_precision prec,
+                               double *qt, uint8_t *ead, uint8_t *long_qt, uint8_t *rechecked, double *qt_err,
+                               lqt_sweep_stats *stats);
+
 /* Host only: QT interval of a pseudo-ECG trace as lqt_strands_results
  * computes it (the reference never defines QT): samples
  * phi[j] at t0 + j*dts; Q at t_q; T peak = largest |phi| in

@@ -68,6 +68,17 @@ class StrandDesc(C.Structure):
     ]
 
 
+class SweepDesc(C.Structure):
+    _fields_ = [("bcl", C.c_double), ("ncell", C.c_int), ("mfrac", C.c_double), ("nbeats", C.c_int),
+                ("qt_threshold", C.c_double), ("band_ms", C.c_double), ("p_lo", C.c_double), ("p_hi", C.c_double),
+                ("dose_max_fmol", C.c_double), ("chunk", C.c_int), ("autotune_steps", C.c_int)]
+
+
+class SweepStats(C.Structure):
+    _fields_ = [("points", C.c_int64), ("rechecked", C.c_int64), ("ead_agree", C.c_double),
+                ("t_surrogate_s", C.c_double), ("t_recheck_s", C.c_double)]
+
+
 class VWDesc(C.Structure):
     _fields_ = [("mode", C.c_int), ("left", C.c_double), ("right", C.c_double), ("step", C.c_double),
                 ("boundary", C.c_double), ("batch", C.c_int), ("max_candidates", C.c_int64)]
@@ -170,6 +181,8 @@ def lib():
     _sig(L, "lqt_mlp_forward", [_VP, C.c_int64, _VP, _VP])
     _sig(L, "lqt_mlp_grads", [_VP, C.c_int, _VP, _VP, _VP, _VP] + [_VP] * 6)
     _sig(L, "lqt_mlp_train", [_VP, C.c_int64, _VP, _VP, _VP, C.c_int, C.c_int, C.c_double, C.c_uint64, _VP])
+    _sig(L, "lqt_surrogate_sweep", [_VP, C.c_int, _VP, C.c_int, _VP, C.c_int, _VP, P(SweepDesc), C.c_int, _VP, _VP,
+                                    _VP, _VP, _VP, P(SweepStats)])
     _sig(L, "lqt_ecg_qt", [_VP, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                            P(C.c_double)])
     _lib = L

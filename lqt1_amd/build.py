@@ -31,6 +31,7 @@ SOURCES = [
     "api_strands.cpp",
     "api_mlp.cpp",
     "api_vw.cpp",
+    "api_surrogate.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

@@ -179,26 +179,22 @@ class SweepResult:
 
 
 def sweep(mlp: MLP, iks_grid, dose_grid, iso_values=(0, 1), bcl=1000.0, length=100, mfrac=0.35,
-          qt_threshold=450.0, band_ms=10.0, p_band=(0.2, 0.8), nbeats=10):
+          qt_threshold=450.0, band_ms=10.0, p_band=(0.2, 0.8), nbeats=10, precision=L.FP32):
     """Surrogate over the grid, then the solver on every point within band_ms
-    of the QT threshold or with an EAD probability inside p_band."""
-    I, D, S = np.meshgrid(np.asarray(iks_grid, float), np.asarray(dose_grid, float), np.asarray(iso_values, float),
-                          indexing="ij")
-    flat = np.stack([I.ravel(), S.ravel(), D.ravel()], 1)
-    t0 = time.perf_counter()
-    x = features(flat[:, 0], flat[:, 1], flat[:, 2], bcl, length, mfrac)
-    qt, _, p = decode(mlp.forward(x))
-    t_sur = time.perf_counter() - t0
-    near = (np.abs(qt - qt_threshold) < band_ms) | ((p > p_band[0]) & (p < p_band[1]))
-    pts = [Point(float(a), int(s), float(d), bcl, length, mfrac) for a, s, d in flat[near]]
-    res, t_chk = simulate(pts, nbeats) if pts else ([], 0.0)
-    qt_f = qt.copy()
-    ead_f = p >= 0.5
-    err = np.zeros(len(res))
-    agree = 0
-    for k, (i, r) in enumerate(zip(np.flatnonzero(near), res)):
-        err[k] = qt[i] - r["qt"] if r["qt"] >= 0 else np.nan
-        agree += int((p[i] >= 0.5) == bool(r["ead"]))
-        qt_f[i] = r["qt"]
-        ead_f[i] = bool(r["ead"])
-    return SweepResult(qt_f, ead_f, qt_f > qt_threshold, near, err, agree / max(1, len(res)), t_sur, t_chk)
+    of the QT threshold or with an EAD probability inside p_band
+    (lqt_surrogate_sweep).  Grid order: iks outer, dose, iso inner."""
+    iks = np.ascontiguousarray(iks_grid, np.float64)
+    dose = np.ascontiguousarray(dose_grid, np.float64)
+    iso = np.ascontiguousarray(iso_values, np.int32)
+    n = iks.size * dose.size * iso.size
+    qt, err = np.zeros(n), np.zeros(n)
+    ead, long_qt, near = np.zeros(n, np.uint8), np.zeros(n, np.uint8), np.zeros(n, np.uint8)
+    d = L.SweepDesc(float(bcl), int(length), float(mfrac), int(nbeats), float(qt_threshold), float(band_ms),
+                    float(p_band[0]), float(p_band[1]), DOSE_MAX_FMOL, 4096, 2000)
+    stats = L.SweepStats()
+    L.check(L.lib().lqt_surrogate_sweep(mlp.h, iks.size, L.ptr(iks), dose.size, L.ptr(dose), iso.size, L.ptr(iso),
+                                        C.byref(d), precision, L.ptr(qt), L.ptr(ead), L.ptr(long_qt), L.ptr(near),
+                                        L.ptr(err), C.byref(stats)))
+    near = near.astype(bool)
+    return SweepResult(qt, ead.astype(bool), long_qt.astype(bool), near, err[near], stats.ead_agree,
+                       stats.t_surrogate_s, stats.t_recheck_s)
