@@ -18,7 +18,7 @@ def lqt():
     import lqt1_amd
 
     if lqt1_amd.device_count() < 1:
-        pytest.fail("no HIP device: GPU tests must run on the GPU box")
+        pytest.fail("no HIP device: the tests marked gpu need an MI355X")
     return lqt1_amd
 
 
