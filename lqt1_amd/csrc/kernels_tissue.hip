@@ -213,18 +213,20 @@ __global__ __launch_bounds__(256) void fill_kernel(R* __restrict__ p, int64_t n,
     if (i < n) p[i] = value;
 }
 
-// Plain streaming copy (16 B per lane): the reference point of the HBM
-// counters and of achievable bandwidth.
-__global__ __launch_bounds__(256) void copy_kernel(float4* __restrict__ dst, const float4* __restrict__ src,
+// Streaming copy (16 B per lane, nontemporal loads and stores, the
+// fastest of the simple forms measured on MI355X): the reference point of
+// the HBM counters and of achievable bandwidth.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void copy_kernel(f32x4* __restrict__ dst, const f32x4* __restrict__ src,
                                                    int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i];
+    if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
 void launch_copy(void* dst, const void* src, int64_t bytes, hipStream_t st) {
     const int64_t n = bytes / 16;
     const int B = 256;
-    copy_kernel<<<(n + B - 1) / B, B, 0, st>>>((float4*)dst, (const float4*)src, n);
+    copy_kernel<<<(n + B - 1) / B, B, 0, st>>>((f32x4*)dst, (const f32x4*)src, n);
 }
 
 const TissueVariant kTissueVariants[kNumTissueVariants] = {
