@@ -220,7 +220,7 @@ typedef struct {
     int nstim;            /* cells 0..nstim-1 are stimulated                */
     int rec_every;        /* pseudo-ECG sample every rec_every steps        */
     double rec_from;      /* ms; ECG recorded for time >= rec_from          */
-    double x0;            /* electrode position (mm from cell 0)            */
+    double x0;            /* electrode position (mm from cell 0), on no cell */
     int measure_beat;     /* beat k = floor(t/bcl) in which act/APD/EAD are taken */
     int vrec_every;       /* V of every cell every vrec_every steps from rec_from
                              (the output file of OneD.cc:402-415); 0 = off     */

@@ -319,6 +319,10 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
                   "strand %d: bad dx, bcl, stop_time, rec_every or nstim", i);
         LQT_CHECK(d.s2_duration >= 0 && d.s2_first >= 0 && d.s2_count >= 0, LQT_ERR_ARG,
                   "strand %d: bad S2 duration or cells", i);
+        // the pseudo-ECG weight 1/(x_j - x0)^2 needs the electrode off every cell
+        bool on_cell = false;
+        for (int j = 0; j < d.ncell && d.rec_every > 0; ++j) on_cell |= (j * d.dx - d.x0) * (j * d.dx - d.x0) == 0.0;
+        LQT_CHECK(!on_cell, LQT_ERR_ARG, "strand %d: pseudo-ECG electrode x0 = %g mm lies on a cell", i, d.x0);
         ncells = std::max<int64_t>(ncells, (int64_t)d.cell_offset + d.ncell);
         maxl = std::max(maxl, d.ncell);
     }
@@ -637,24 +641,24 @@ lqt_status lqt_strands_final_state(lqt_strands* s, int i, double* out) {
     LQT_CHECK(s && out && i >= 0 && i < s->n, LQT_ERR_ARG, "lqt_strands_final_state: bad argument");
     LQT_CHECK(s->ran, LQT_ERR_STATE, "lqt_strands_final_state: call lqt_strands_run first");
     const lqt_strand_desc& d = s->desc[i];
+    const int64_t c0 = d.cell_offset, n = d.ncell;
     const size_t rs = rsz(s);
-    for (int j = 0; j < d.ncell; ++j) {
-        const int64_t c = (int64_t)d.cell_offset + j;
-        for (int m = 0; m < 40; ++m) {
-            const Ref40 r = kRef40[m];
-            double val = 1.0;
-            unsigned char buf[8];
-            if (r.kind == 0) {
-                LQT_HIP(hipMemcpy(buf, (unsigned char*)s->fv + rs * c, rs, hipMemcpyDeviceToHost));
-                val = s->fp32 ? (double)*(float*)buf : *(double*)buf;
-            } else if (r.kind == 1) {
-                LQT_HIP(hipMemcpy(buf, (unsigned char*)s->fr + rs * ((int64_t)r.slot * s->pitch + c), rs,
-                                  hipMemcpyDeviceToHost));
-                val = s->fp32 ? (double)*(float*)buf : *(double*)buf;
-            } else if (r.kind == 2) {
-                LQT_HIP(hipMemcpy(&val, s->fd + (int64_t)r.slot * s->pitch + c, sizeof(double), hipMemcpyDeviceToHost));
-            }
-            out[(size_t)j * 40 + m] = val;
+    // the strand's cells are contiguous in every slot: one copy per slot
+    std::vector<unsigned char> buf(8 * (size_t)n);
+    for (int m = 0; m < 40; ++m) {
+        const Ref40 r = kRef40[m];
+        if (r.kind == 0 || r.kind == 1) {
+            const unsigned char* src = r.kind == 0 ? (const unsigned char*)s->fv + rs * c0
+                                                   : (const unsigned char*)s->fr + rs * ((int64_t)r.slot * s->pitch + c0);
+            LQT_HIP(hipMemcpy(buf.data(), src, rs * n, hipMemcpyDeviceToHost));
+            for (int64_t j = 0; j < n; ++j)
+                out[(size_t)j * 40 + m] = s->fp32 ? (double)((const float*)buf.data())[j] : ((const double*)buf.data())[j];
+        } else if (r.kind == 2) {
+            LQT_HIP(hipMemcpy(buf.data(), s->fd + (int64_t)r.slot * s->pitch + c0, sizeof(double) * n,
+                              hipMemcpyDeviceToHost));
+            for (int64_t j = 0; j < n; ++j) out[(size_t)j * 40 + m] = ((const double*)buf.data())[j];
+        } else {
+            for (int64_t j = 0; j < n; ++j) out[(size_t)j * 40 + m] = 1.0;  // hsp, jp: never updated
         }
     }
     return LQT_OK;

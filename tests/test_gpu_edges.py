@@ -6,8 +6,8 @@
   FP32, isotropic and anisotropic.
 * Strands at the wavefront edges (2, 63, 64, 65 and 128 cells) against the
   FP64 oracle of OneD.cc, in the per-size and the packed layouts.
-* Empty inputs, and strands outside the documented 2..512 cells, are
-  refused with an error, not run.
+* Empty inputs, strands outside the documented 2..512 cells and a
+  pseudo-ECG electrode on a cell are refused with an error, not run.
 """
 import numpy as np
 import pytest
@@ -91,3 +91,8 @@ def test_empty_inputs_are_refused(lqt):
             lqt.Strands([lqt.StrandSpec(lqt.transmural(n), lqt.preset("cpu_default"), nbeats=1)])
     with pytest.raises(lqt.LqtError):
         lqt.Cells(np.zeros(0, np.uint8))
+    # a pseudo-ECG electrode on a cell (here cell 0) would weight it by 1/0
+    n = 20
+    with pytest.raises(lqt.LqtError, match="electrode"):
+        lqt.Strands([lqt.StrandSpec(lqt.transmural(n), lqt.preset("cpu_default"), nbeats=1,
+                                    electrode_mm=-(n * 0.15))])
