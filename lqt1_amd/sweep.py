@@ -10,9 +10,10 @@
    accumulation, Adam).
 3. Sweep: grid x grid IKs scale x dose at ISO off/on (BCL 1000 ms, 100 cells,
    M fraction 0.35); every point within 10 ms of the QT threshold or with an
-   EAD probability in (0.2, 0.8) is re-run by the solver.
-4. With `--plain`, the whole grid is also run by the solver, and the
-   steered decisions are compared with it point by point.
+   EAD probability in (0.2, 0.8) is re-run by the solver.  One sweep per QT
+   threshold (`--qt-threshold`, or `--qt-percentiles` of the training QT).
+4. With `--plain`, the whole grid is also run by the solver once, and the
+   steered decisions of every threshold are compared with it point by point.
 """
 from __future__ import annotations
 
@@ -34,7 +35,9 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=200)
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--qt-threshold", type=float, default=None, help="default: median training QT at BCL 1000")
+    ap.add_argument("--qt-threshold", type=float, default=None, help="one QT threshold (ms); default: --qt-percentiles")
+    ap.add_argument("--qt-percentiles", default="50,75,90,97.5",
+                    help="QT thresholds as percentiles of the training QT at BCL 1000 ms, one sweep each")
     ap.add_argument("--plain", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--json", default=None)
@@ -75,37 +78,49 @@ def main(argv=None):
     out["heldout_ead_acc"] = float(np.mean((p_p >= 0.5) == (hy[:, S.N_REG] > 0.5)))
     log(f"held-out: QT MAE {out['heldout_qt_mae_ms']:.2f} ms, EAD accuracy {out['heldout_ead_acc']:.3f}")
 
-    thr = a.qt_threshold
-    if thr is None:
-        q = np.array([r["qt"] for r, p in zip(res, pts) if p.bcl == 1000.0 and r["qt"] > 0])
-        thr = float(np.median(q)) if q.size else 450.0
+    # QT thresholds: the given one, else percentiles of the training QT at
+    # BCL 1000 ms (the median puts the threshold where QT values are
+    # densest, so its band holds the most grid points)
+    q = np.array([r["qt"] for r, p in zip(res, pts) if p.bcl == 1000.0 and r["qt"] > 0])
+    if a.qt_threshold is not None:
+        thrs = [(None, a.qt_threshold)]
+    else:
+        pcts = [float(x) for x in a.qt_percentiles.split(",")]
+        thrs = [(pc, float(np.percentile(q, pc)) if q.size else 450.0) for pc in pcts]
     g = np.linspace(0.0, 1.0, a.grid)
-    sw = S.sweep(net, g, g, (0, 1), qt_threshold=thr, band_ms=10.0, nbeats=a.beats)
-    n_pts = sw.qt.size
-    out.update(qt_threshold=thr, grid_points=int(n_pts), rechecked=int(sw.rechecked.sum()),
-               t_surrogate_s=sw.t_surrogate, t_recheck_s=sw.t_recheck,
-               recheck_qt_mae_ms=float(np.nanmean(np.abs(sw.qt_err))) if sw.qt_err.size else 0.0,
-               recheck_ead_agree=sw.ead_agree)
-    log(f"sweep: {n_pts} points, surrogate {sw.t_surrogate * 1e3:.1f} ms, re-checked {sw.rechecked.sum()} "
-        f"in {sw.t_recheck:.2f} s")
+    plain = None
     if a.plain:
         I, D, SS = np.meshgrid(g, g, [0.0, 1.0], indexing="ij")
         grid = [S.Point(float(i), int(s), float(d), 1000.0, 100, 0.35) for i, d, s in zip(I.ravel(), D.ravel(), SS.ravel())]
         t0 = time.perf_counter()
-        plain, t_plain_dev = S.simulate(grid, nbeats=a.beats, progress=prog)
+        pres, _ = S.simulate(grid, nbeats=a.beats, progress=prog)
         t_plain = time.perf_counter() - t0
-        qt_p = np.array([r["qt"] for r in plain])
-        ead_p = np.array([bool(r["ead"]) for r in plain])
-        chk = sw.rechecked
-        out.update(t_plain_s=t_plain,
-                   speedup=t_plain / (sw.t_surrogate + sw.t_recheck),
-                   rechecked_identical=bool(np.array_equal(sw.long_qt[chk], qt_p[chk] > thr) and
-                                            np.array_equal(sw.ead[chk], ead_p[chk])),
-                   agree_long_qt=float(np.mean(sw.long_qt == (qt_p > thr))),
-                   agree_ead=float(np.mean(sw.ead == ead_p)))
-        log(f"plain sweep {t_plain:.1f} s -> speed-up {out['speedup']:.1f}x, decisions agree "
-            f"{out['agree_long_qt']:.4f} (QT) {out['agree_ead']:.4f} (EAD), re-checked identical "
-            f"{out['rechecked_identical']}")
+        plain = (np.array([r["qt"] for r in pres]), np.array([bool(r["ead"]) for r in pres]))
+        out["t_plain_s"] = t_plain
+        log(f"plain sweep of {len(grid)} points: {t_plain:.1f} s")
+    runs = []
+    for pc, thr in thrs:
+        sw = S.sweep(net, g, g, (0, 1), qt_threshold=thr, band_ms=10.0, nbeats=a.beats)
+        n_pts = sw.qt.size
+        r = dict(qt_percentile=pc, qt_threshold=thr, grid_points=int(n_pts), rechecked=int(sw.rechecked.sum()),
+                 t_surrogate_s=sw.t_surrogate, t_recheck_s=sw.t_recheck,
+                 recheck_qt_mae_ms=float(np.nanmean(np.abs(sw.qt_err))) if np.isfinite(sw.qt_err).any() else 0.0,
+                 recheck_ead_agree=sw.ead_agree)
+        log(f"sweep at QT threshold {thr:.1f} ms: {n_pts} points, surrogate {sw.t_surrogate * 1e3:.1f} ms, "
+            f"re-checked {sw.rechecked.sum()} in {sw.t_recheck:.2f} s")
+        if plain is not None:
+            qt_p, ead_p = plain
+            chk = sw.rechecked
+            r.update(speedup=out["t_plain_s"] / (sw.t_surrogate + sw.t_recheck),
+                     rechecked_identical=bool(np.array_equal(sw.long_qt[chk], qt_p[chk] > thr) and
+                                              np.array_equal(sw.ead[chk], ead_p[chk])),
+                     agree_long_qt=float(np.mean(sw.long_qt == (qt_p > thr))),
+                     agree_ead=float(np.mean(sw.ead == ead_p)))
+            log(f"  speed-up {r['speedup']:.1f}x, decisions agree {r['agree_long_qt']:.4f} (QT) "
+                f"{r['agree_ead']:.4f} (EAD), re-checked identical {r['rechecked_identical']}")
+        runs.append(r)
+    out.update(runs[0])  # the first threshold, flat, as before
+    out["thresholds"] = runs
     print(json.dumps(out), flush=True)
     if a.json:
         with open(a.json, "w") as f:
