@@ -54,6 +54,41 @@ def tissue_run(A, geo, prec, warmup, steps, tune_steps=20):
     return out
 
 
+def accuracy_3d(A, geo, t_end=1000.0):
+    """Per-voxel activation time and APD90 of the ventricle's first beat in
+    FP32 and in FP64 from identical inputs (the tissue's own trackers over
+    [0, t_end) ms), compared voxel by voxel."""
+    import numpy as np
+
+    maps = {}
+    for name, prec in (("fp32", A.FP32), ("fp64", A.FP64)):
+        T = A.Tissue(geo.label, geo.fibre, geo.dx, 0.154, 0.0385, A.preset("config3"), prec, 0.02)
+        T.set_stimulus(geo.stim_sites, geo.stim_offsets, bcl=1000.0, duration=3.0, strength=-52.0)
+        T.track(0.0, t_end)
+        t, n = 0.0, 0
+        while t < t_end:  # the tissue clock's own accumulator
+            t += 0.02
+            n += 1
+        T.step(n)
+        maps[name] = T.track_results()
+        T.close()
+    (a32, p32), (a64, p64) = maps["fp32"], maps["fp64"]
+    both = (a32 >= 0) & (a64 >= 0)
+    rep = both & (p32 >= 0) & (p64 >= 0)
+    d_act = np.abs(a32[both] - a64[both])
+    d_apd = np.abs(p32[rep] - p64[rep])
+    return dict(window_ms=t_end, voxels=int(a64.size), activated=int(both.sum()),
+                activated_one_only=int(((a32 >= 0) != (a64 >= 0)).sum()), repolarised=int(rep.sum()),
+                act_fp64_ms=[float(a64[both].min()), float(a64[both].max())] if both.any() else None,
+                apd90_fp64_ms_min_p1_p50_p99_max=[float(x) for x in np.percentile(p64[rep], [0, 1, 50, 99, 100])]
+                if rep.any() else None,
+                d_act_ms_max=float(d_act.max()) if d_act.size else None,
+                d_act_steps_over_1=int((d_act > 0.02 + 1e-9).sum()),
+                d_apd90_ms_max=float(d_apd.max()) if d_apd.size else None,
+                d_apd90_ms_p99=float(np.percentile(d_apd, 99)) if d_apd.size else None,
+                d_apd90_over_1ms=int((d_apd > 1.0).sum()))
+
+
 def cpu_baseline(geo, steps, threads):
     """The FP64 oracle's tissue step (OpenMP over voxels) on the same
     geometry for a bounded number of steps."""
@@ -244,6 +279,8 @@ def main():
     ap.add_argument("--tune-steps", type=int, default=20, help="autotuner trial steps per launch shape (0 = off)")
     ap.add_argument("--no-strands", action="store_true")
     ap.add_argument("--no-sheet", action="store_true", help="skip the 2D annulus sheet leg")
+    ap.add_argument("--no-accuracy3d", action="store_true",
+                    help="skip the per-voxel FP32 vs FP64 activation / APD90 comparison over the first beat")
     ap.add_argument("--strand-beats", type=int, default=10, help="beats per strand of the 512-strand sweep")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--profile", choices=["fp32", "fp64", "copy", "cells", "strands", "strand-latency", "sheet"],
@@ -335,6 +372,13 @@ def main():
         acc.update(strand_accuracy(A))
     except Exception as e:  # reported, not hidden
         acc["strand_error"] = str(e)
+    if not args.no_accuracy3d:
+        try:
+            t0 = time.perf_counter()
+            acc["ventricle_first_beat"] = accuracy_3d(A, geo)
+            log(f"3D accuracy ({time.perf_counter() - t0:.1f} s): {acc['ventricle_first_beat']}")
+        except Exception as e:
+            acc["ventricle_error"] = str(e)
     achieved = r32["bytes"] / (r32["us_main"] * 1e-6) / 1e9
     traffic = measured_traffic(r32["shape"])
     # the practical streaming ceiling of this GPU: a plain 2 GiB copy

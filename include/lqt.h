@@ -171,6 +171,18 @@ lqt_status lqt_tissue_write_state(lqt_tissue *t, int64_t k, const double *in);
 /* Counts of V < -90 mV, V > 200 mV and non-finite V (GPU3D.cu:1796-1829),
  * reduced on the device. */
 lqt_status lqt_tissue_v_guard(lqt_tissue *t, int64_t counts[3]);
+/* Activation and APD90 maps.  While enabled, every step that starts at a
+ * tissue-clock time t in [t_from, t_to) updates a tracker per voxel with the
+ * voxel's new V: activation time = t of the first step whose V >= -30 mV
+ * (the criterion of OneD.cc:422), APD90 = time from activation to the first
+ * later step whose V <= Vmax - 0.9 (Vmax - Vrest), with Vmax the peak since
+ * activation and Vrest the V after the window's first step.  No counterpart
+ * in the reference's tissue programs, which write V maps only.  Calling
+ * lqt_tissue_track resets the trackers; t_to <= t_from switches them off. */
+lqt_status lqt_tissue_track(lqt_tissue *t, double t_from, double t_to);
+/* act[nvox], apd90[nvox] in ms (either may be NULL), -1 where the voxel has
+ * not activated / not repolarised within the window. */
+lqt_status lqt_tissue_track_results(lqt_tissue *t, double *act, double *apd90);
 /* Legacy VTK STRUCTURED_POINTS of V on every `stride`-th voxel, -100 outside
  * (GPU3D.cu:123-163 writes stride 3). */
 lqt_status lqt_tissue_write_vtk(lqt_tissue *t, const char *path, int stride);

@@ -149,6 +149,8 @@ def lib():
     _sig(L, "lqt_tissue_read_state", [_VP, C.c_int64, _VP])
     _sig(L, "lqt_tissue_write_state", [_VP, C.c_int64, _VP])
     _sig(L, "lqt_tissue_v_guard", [_VP, _VP])
+    _sig(L, "lqt_tissue_track", [_VP, C.c_double, C.c_double])
+    _sig(L, "lqt_tissue_track_results", [_VP, _VP, _VP])
     _sig(L, "lqt_tissue_write_vtk", [_VP, C.c_char_p, C.c_int])
     _sig(L, "lqt_tissue_save", [_VP, C.c_char_p])
     _sig(L, "lqt_tissue_load", [_VP, C.c_char_p])

@@ -41,6 +41,11 @@ struct lqt_tissue {
     float* stim_off = nullptr;
     uint8_t* s2 = nullptr;
     unsigned long long* guard = nullptr;
+    // per-voxel activation / APD90 trackers, updated after each step that
+    // starts in [trk_from, trk_to)
+    VoxTrack* track = nullptr;
+    bool trk_on = false;
+    double trk_from = 0, trk_to = 0;
     // stimulus
     lqt_stim_protocol proto{};
     int64_t nsites = 0, ns2 = 0;
@@ -430,6 +435,7 @@ lqt_status lqt_tissue_destroy(lqt_tissue* t) {
     hipFree(t->stim_off);
     hipFree(t->s2);
     hipFree(t->guard);
+    hipFree(t->track);
     for (hipEvent_t e : t->ev) hipEventDestroy(e);
     if (t->stream) hipStreamDestroy(t->stream);
     delete t;
@@ -553,6 +559,8 @@ lqt_status lqt_tissue_step_async(lqt_tissue* t, int64_t nsteps, void* stream) {
             launch_tissue_step(false, t->aniso, t->nz == 1, t->variant, &a, t->nvox, s);
         }
         if (timed) LQT_HIP(hipEventRecord(t->ev[2 * ti + 1], s));
+        if (t->trk_on && t->time >= t->trk_from && t->time < t->trk_to)
+            launch_track(t->fp32, t->nvox, t->vpos, t->grid[t->cur ^ 1], t->time, t->track, s);
         LQT_HIP(hipGetLastError());
         t->cur ^= 1;
         t->time += t->dt;  // GPU3D.cu:1840 double accumulator
@@ -743,6 +751,37 @@ lqt_status lqt_tissue_autotune(lqt_tissue* t, int trial_steps, int* best, double
     t->variant = st ? keep : ibest;
     if (best) *best = t->variant;
     return st;
+}
+
+lqt_status lqt_tissue_track(lqt_tissue* t, double t_from, double t_to) {
+    LQT_CHECK(t, LQT_ERR_ARG, "lqt_tissue_track: null handle");
+    LQT_HIP(hipStreamSynchronize(t->stream));
+    if (!(t_to > t_from)) {
+        t->trk_on = false;
+        return LQT_OK;
+    }
+    if (!t->track) LQT_HIP(hipMalloc((void**)&t->track, sizeof(VoxTrack) * std::max<int64_t>(1, t->nvox)));
+    launch_track_reset(t->nvox, t->track, t->stream);
+    LQT_HIP(hipGetLastError());
+    LQT_HIP(hipStreamSynchronize(t->stream));
+    t->trk_on = true;
+    t->trk_from = t_from;
+    t->trk_to = t_to;
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_track_results(lqt_tissue* t, double* act, double* apd90) {
+    LQT_CHECK(t && (act || apd90), LQT_ERR_ARG, "lqt_tissue_track_results: bad argument");
+    LQT_CHECK(t->track, LQT_ERR_STATE, "lqt_tissue_track_results: call lqt_tissue_track first");
+    LQT_HIP(hipStreamSynchronize(t->stream));
+    std::vector<VoxTrack> h(t->nvox);
+    LQT_HIP(hipMemcpy(h.data(), t->track, sizeof(VoxTrack) * t->nvox, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < t->nvox; ++k) {
+        const VoxTrack& v = h[k];
+        if (act) act[k] = (v.flags & 2) ? v.act : -1.0;
+        if (apd90) apd90[k] = (v.flags & 4) ? v.rep - v.act : -1.0;
+    }
+    return LQT_OK;
 }
 
 lqt_status lqt_tissue_last_kernel_us(const lqt_tissue* t, double* us_step_kernel, double* us_ghost_kernel) {

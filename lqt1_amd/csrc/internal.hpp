@@ -31,6 +31,15 @@ struct PaceArgs {
     double* trace;      // device, or nullptr
 };
 
+// Per-voxel beat tracker of the tissue (activation and APD90 within a
+// window of the tissue clock), updated after each step while enabled.
+struct VoxTrack {
+    double act, rep;     // step times (ms) of the -30 mV up-crossing and of 90 % repolarisation
+    double vrest, vmax;  // V at the window's first step; peak V since activation
+    int flags;           // bit0 window entered, bit1 activated, bit2 repolarised
+    int pad;
+};
+
 template <class R>
 struct TissueArgs {
     int64_t nvox;
@@ -82,6 +91,9 @@ void launch_tissue_step(bool fp32, bool aniso, bool flat, int variant, const voi
 void launch_ghost_fill(bool fp32, void* grid, const int32_t* dst, const int32_t* src, int64_t n,
                        hipStream_t st);
 void launch_fill(bool fp32, void* p, int64_t n, double value, hipStream_t st);
+void launch_track(bool fp32, int64_t nvox, const int32_t* vpos, const void* grid, double time, VoxTrack* tr,
+                  hipStream_t st);
+void launch_track_reset(int64_t nvox, VoxTrack* tr, hipStream_t st);
 void launch_copy(void* dst, const void* src, int64_t bytes, hipStream_t st);
 
 }  // namespace lqt

@@ -213,6 +213,51 @@ __global__ __launch_bounds__(256) void fill_kernel(R* __restrict__ p, int64_t n,
     if (i < n) p[i] = value;
 }
 
+// Beat tracker of every voxel after a step that started at `time`: the
+// rules of the strand tracker (activation at the first new V >= -30 mV,
+// OneD.cc:422; 90 % repolarisation against the peak since activation and V
+// at the window's first step).
+template <class R>
+__global__ __launch_bounds__(256) void track_kernel(int64_t nvox, const int32_t* __restrict__ vpos,
+                                                     const R* __restrict__ grid, double time,
+                                                     VoxTrack* __restrict__ tr) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nvox) return;
+    const double v = (double)grid[vpos[k]];
+    VoxTrack t = tr[k];
+    if (!(t.flags & 1)) {
+        t.flags |= 1;
+        t.vrest = v;
+    }
+    if (!(t.flags & 2)) {
+        if (v >= -30.0) {
+            t.flags |= 2;
+            t.act = time;
+            t.vmax = v;
+        }
+    } else if (!(t.flags & 4)) {
+        if (v > t.vmax) t.vmax = v;
+        if (v <= t.vmax - 0.9 * (t.vmax - t.vrest)) {
+            t.flags |= 4;
+            t.rep = time;
+        }
+    }
+    tr[k] = t;
+}
+
+__global__ __launch_bounds__(256) void track_reset_kernel(int64_t nvox, VoxTrack* __restrict__ tr) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nvox) return;
+    VoxTrack t;
+    t.act = -1.0;
+    t.rep = -1.0;
+    t.vrest = 0.0;
+    t.vmax = 0.0;
+    t.flags = 0;
+    t.pad = 0;
+    tr[k] = t;
+}
+
 // Streaming copy (16 B per lane, nontemporal loads and stores, the
 // fastest of the simple forms measured on MI355X): the reference point of
 // the HBM counters and of achievable bandwidth.
@@ -277,6 +322,21 @@ void launch_ghost_fill(bool fp32, void* grid, const int32_t* dst, const int32_t*
     const int64_t G = (n + B - 1) / B;
     if (fp32) ghost_fill_kernel<float><<<G, B, 0, st>>>((float*)grid, dst, src, n);
     else ghost_fill_kernel<double><<<G, B, 0, st>>>((double*)grid, dst, src, n);
+}
+
+void launch_track(bool fp32, int64_t nvox, const int32_t* vpos, const void* grid, double time, VoxTrack* tr,
+                  hipStream_t st) {
+    if (nvox == 0) return;
+    const int B = 256;
+    const int64_t G = (nvox + B - 1) / B;
+    if (fp32) track_kernel<float><<<G, B, 0, st>>>(nvox, vpos, (const float*)grid, time, tr);
+    else track_kernel<double><<<G, B, 0, st>>>(nvox, vpos, (const double*)grid, time, tr);
+}
+
+void launch_track_reset(int64_t nvox, VoxTrack* tr, hipStream_t st) {
+    if (nvox == 0) return;
+    const int B = 256;
+    track_reset_kernel<<<(nvox + B - 1) / B, B, 0, st>>>(nvox, tr);
 }
 
 void launch_fill(bool fp32, void* p, int64_t n, double value, hipStream_t st) {

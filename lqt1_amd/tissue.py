@@ -69,6 +69,17 @@ class Tissue:
         L.check(L.lib().lqt_tissue_v_guard(self.h, L.ptr(out)))
         return out
 
+    def track(self, t_from=0.0, t_to=1e300):
+        """Start (reset) the activation / APD90 trackers for steps starting
+        in [t_from, t_to) ms of the tissue clock; t_to <= t_from stops them."""
+        L.check(L.lib().lqt_tissue_track(self.h, float(t_from), float(t_to)))
+
+    def track_results(self):
+        """(activation time, APD90) per voxel in ms, -1 where none."""
+        act, apd = np.zeros(self.nvox), np.zeros(self.nvox)
+        L.check(L.lib().lqt_tissue_track_results(self.h, L.ptr(act), L.ptr(apd)))
+        return act, apd
+
     def kernel_us(self):
         a, b = C.c_double(), C.c_double()
         L.check(L.lib().lqt_tissue_last_kernel_us(self.h, C.byref(a), C.byref(b)))
