@@ -79,6 +79,7 @@ def accuracy_3d(A, geo, t_end=1000.0):
     d_apd = np.abs(p32[rep] - p64[rep])
     return dict(window_ms=t_end, voxels=int(a64.size), activated=int(both.sum()),
                 activated_one_only=int(((a32 >= 0) != (a64 >= 0)).sum()), repolarised=int(rep.sum()),
+                repolarised_one_only=int((both & ((p32 >= 0) != (p64 >= 0))).sum()),
                 act_fp64_ms=[float(a64[both].min()), float(a64[both].max())] if both.any() else None,
                 apd90_fp64_ms_min_p1_p50_p99_max=[float(x) for x in np.percentile(p64[rep], [0, 1, 50, 99, 100])]
                 if rep.any() else None,

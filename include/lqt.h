@@ -27,7 +27,12 @@ typedef int lqt_status;
 #define LQT_ERR_RANGE 4    /* non-finite membrane potential detected      */
 #define LQT_ERR_STATE 5    /* call out of order                           */
 
+/* LQT_FP32 keeps gates, V and the fast concentrations in float and the slow
+ * concentrations in double; its handles accept dt up to LQT_FP32_MAX_DT ms
+ * (the slow gates' Rush-Larsen step uses a series valid to dt/tau = 0.125,
+ * and every such gate has tau >= 0.8 ms).  LQT_FP64 takes any dt > 0. */
 typedef enum { LQT_FP64 = 0, LQT_FP32 = 1 } lqt_precision;
+#define LQT_FP32_MAX_DT 0.1
 
 /* enum CellType of SingleCell/Cell.cc:20 (values kept).  In a tissue label
  * grid, LQT_HOLD may be OR'ed onto a type: the voxel is updated as that

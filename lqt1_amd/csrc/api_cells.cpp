@@ -205,6 +205,8 @@ lqt_status lqt_cells_create(int n, const uint8_t* celltype, const lqt_params* pa
     LQT_CHECK(n > 0 && celltype && params && out, LQT_ERR_ARG, "lqt_cells_create: bad argument");
     LQT_CHECK(nparams == 1 || nparams == n, LQT_ERR_ARG, "lqt_cells_create: nparams must be 1 or n");
     LQT_CHECK(dt > 0, LQT_ERR_ARG, "lqt_cells_create: dt must be > 0");
+    LQT_CHECK(prec == LQT_FP64 || dt <= LQT_FP32_MAX_DT, LQT_ERR_ARG, "lqt_cells_create: FP32 needs dt <= %g ms",
+              LQT_FP32_MAX_DT);
     LQT_CHECK(prec == LQT_FP32 || prec == LQT_FP64, LQT_ERR_ARG, "lqt_cells_create: bad precision");
     int ndev = 0;
     LQT_CHECK(hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0, LQT_ERR_HIP,

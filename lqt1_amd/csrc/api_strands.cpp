@@ -305,6 +305,8 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
                               double dt, lqt_strands** out) {
     LQT_CHECK(nstrands > 0 && desc && celltype && params && nparams > 0 && out && dt > 0, LQT_ERR_ARG,
               "lqt_strands_create: bad argument");
+    LQT_CHECK(prec == LQT_FP64 || dt <= LQT_FP32_MAX_DT, LQT_ERR_ARG, "lqt_strands_create: FP32 needs dt <= %g ms",
+              LQT_FP32_MAX_DT);
     int ndev = 0;
     LQT_CHECK(hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0, LQT_ERR_HIP,
               "lqt_strands_create: no HIP device available");

@@ -269,6 +269,8 @@ lqt_status lqt_tissue_create(const lqt_geometry* geo, const lqt_params* params, 
     LQT_CHECK(geo && params && out && geo->label, LQT_ERR_ARG, "lqt_tissue_create: null argument");
     LQT_CHECK(geo->nx > 0 && geo->ny > 0 && geo->nz > 0 && geo->dx > 0 && dt > 0, LQT_ERR_ARG,
               "lqt_tissue_create: bad dimensions or step");
+    LQT_CHECK(prec == LQT_FP64 || dt <= LQT_FP32_MAX_DT, LQT_ERR_ARG, "lqt_tissue_create: FP32 needs dt <= %g ms",
+              LQT_FP32_MAX_DT);
     const int64_t ncell = (int64_t)geo->nx * geo->ny * geo->nz;
     const int64_t npad = (int64_t)(geo->nx + 2) * (geo->ny + 2) * (geo->nz + 2);
     LQT_CHECK(npad < (int64_t)INT32_MAX, LQT_ERR_ARG, "lqt_tissue_create: grid too large for 32-bit indices");

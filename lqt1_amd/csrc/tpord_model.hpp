@@ -79,6 +79,17 @@ struct Math<float> {
     // ndt = -dt * log2(e) formed once per update.
     static __device__ __forceinline__ float decay_scale(float dt) { return -dt * 1.4426950408889634f; }
     static __device__ __forceinline__ float decay(float ndt, float rate) { return __builtin_amdgcn_exp2f(rate * ndt); }
+    // Rush-Larsen step of a gate whose time constant is at least 0.8 ms at
+    // every V, for dt <= 0.1 ms: x + (x_inf - x)(1 - exp(-z)), z = dt*rate
+    // <= 0.125, with 1 - exp(-z) from its series to z^5 (truncation below
+    // 4e-8 relative).  Forming 1 - exp(-z) from a rounded exp(-z) near 1
+    // would keep only about log2(z / 6e-8) bits of it (8 bits for tau =
+    // 1000 ms), a systematic error in the slow time constants.
+    static __device__ __forceinline__ float rl_slow(float x, float xinf, float dts, float rate) {
+        const float z = rate * dts;
+        const float em1 = z * (1.0f - z * (0.5f - z * (1.0f / 6.0f - z * (1.0f / 24.0f - z * (1.0f / 120.0f)))));
+        return x + (xinf - x) * em1;
+    }
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float log(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }
@@ -102,6 +113,11 @@ struct Math<double> {
     // exp(-dt * rate): (-dt) * rate rounds exactly as -(dt * rate)
     static __device__ __forceinline__ double decay_scale(double dt) { return -dt; }
     static __device__ __forceinline__ double decay(double ndt, double rate) { return ::exp(rate * ndt); }
+    // in double the exponential form keeps full precision: the same
+    // expression as every other gate, x_inf - (x_inf - x) exp(-dt*rate)
+    static __device__ __forceinline__ double rl_slow(double x, double xinf, double dts, double rate) {
+        return xinf - (xinf - x) * ::exp(rate * -dts);
+    }
     static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
     static __device__ __forceinline__ double div(double a, double b) { return a / b; }
     static __device__ __forceinline__ double log(double x) { return ::log(x); }
@@ -243,9 +259,9 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R tm_rate = R(6.765) * M::exl(v, 1.0 / 34.77, 11.64 / 34.77) + R(8.552) * M::exl(v, -1.0 / 5.955, -77.42 / 5.955);
         s.mL = mLss - (mLss - s.mL) * M::decay(ndt, tm_rate);
         const R hLss = M::rcp(R(1) + M::exl(v, 1.0 / 7.488, 87.61 / 7.488));
-        s.hL = hLss - (hLss - s.hL) * M::decay(ndt, R(1.0 / 200.0));
+        s.hL = M::rl_slow(s.hL, hLss, dt, R(1.0 / 200.0));
         const R hLssp = M::rcp(R(1) + M::exl(v, 1.0 / 7.488, 93.81 / 7.488));
-        s.hLp = hLssp - (hLssp - s.hLp) * M::decay(ndt, R(1.0 / 600.0));
+        s.hLp = M::rl_slow(s.hLp, hLssp, dt, R(1.0 / 600.0));
     }
     const R GNaL = epi ? R(0.0075 * 0.6) : R(0.0075);
     const R INaL = GNaL * (v - ENa) * s.mL * ((R(1) - fCaMKp) * s.hL + fCaMKp * s.hLp);
@@ -271,7 +287,7 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R AiF = M::rcp(R(1) + M::exl(v, 1.0 / 151.2, -213.6 / 151.2));
         const R AiS = R(1) - AiF;
         s.iF = iss - (iss - s.iF) * M::decay(ndt, M::rcp(tiF));
-        s.iS = iss - (iss - s.iS) * M::decay(ndt, M::rcp(tiS));
+        s.iS = M::rl_slow(s.iS, iss, dt, M::rcp(tiS));
         const R i_ = AiF * s.iF + AiS * s.iS;
         const R assp = M::rcp(R(1) + M::exl(v, -1.0 / 14.82, 24.34 / 14.82));
         s.ap = assp - (assp - s.ap) * e_ta;
@@ -280,7 +296,7 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R dti_recover = R(1) - R(0.5) * M::rcp(R(1) + M::exl(v, 1.0 / 20.0, 70.0 / 20.0));
         const R dd = dti_develop * dti_recover;
         s.iFp = iss - (iss - s.iFp) * M::decay(ndt, M::rcp(dd * tiF));
-        s.iSp = iss - (iss - s.iSp) * M::decay(ndt, M::rcp(dd * tiS));
+        s.iSp = M::rl_slow(s.iSp, iss, dt, M::rcp(dd * tiS));
         const R ip = AiF * s.iFp + AiS * s.iSp;
         const R Gto = (epi || mid) ? R(0.08) : R(0.02);
         Ito = Gto * (v - EK) * ((R(1) - fCaMKp) * s.a * i_ + fCaMKp * s.ap * ip);
@@ -302,8 +318,8 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R tfs = R(1000) + M::rcp(R(0.000035) * M::exl(v, -0.25, -1.25) + R(0.000035) * M::exl(v, 1.0 / 6.0, 5.0 / 6.0));
         const R Aff = R(0.6), Afs = R(0.4);
         const R inv_tff = M::rcp(tff);
-        s.ff = fss - (fss - s.ff) * M::decay(ndt, inv_tff);
-        s.fs = fss - (fss - s.fs) * M::decay(ndt, M::rcp(tfs));
+        s.ff = M::rl_slow(s.ff, fss, dt, inv_tff);
+        s.fs = M::rl_slow(s.fs, fss, dt, M::rcp(tfs));
         const R f = Aff * s.ff + Afs * s.fs;
         const R fcass = fss;
         const R tfcaf = R(7) + M::rcp(R(0.04) * M::exl(v, -1.0 / 7.0, 4.0 / 7.0) + R(0.04) * M::exl(v, 1.0 / 7.0, -4.0 / 7.0));
@@ -311,13 +327,13 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R Afcaf = R(0.3) + R(0.6) * M::rcp(R(1) + M::exl(v, 0.1, -1.0));
         const R Afcas = R(1) - Afcaf;
         const R inv_tfcaf = M::rcp(tfcaf);
-        s.fcaf = fcass - (fcass - s.fcaf) * M::decay(ndt, inv_tfcaf);
-        s.fcas = fcass - (fcass - s.fcas) * M::decay(ndt, M::rcp(tfcas));
+        s.fcaf = M::rl_slow(s.fcaf, fcass, dt, inv_tfcaf);
+        s.fcas = M::rl_slow(s.fcas, fcass, dt, M::rcp(tfcas));
         const R fca = Afcaf * s.fcaf + Afcas * s.fcas;
-        s.jca = fcass - (fcass - s.jca) * M::decay(ndt, R(1.0 / 75.0));
-        s.ffp = fss - (fss - s.ffp) * M::decay(ndt * R(1.0 / 2.5), inv_tff);
+        s.jca = M::rl_slow(s.jca, fcass, dt, R(1.0 / 75.0));
+        s.ffp = M::rl_slow(s.ffp, fss, dt * R(1.0 / 2.5), inv_tff);
         const R fp = Aff * s.ffp + Afs * s.fs;
-        s.fcafp = fcass - (fcass - s.fcafp) * M::decay(ndt * R(1.0 / 2.5), inv_tfcaf);
+        s.fcafp = M::rl_slow(s.fcafp, fcass, dt * R(1.0 / 2.5), inv_tfcaf);
         const R fcap = Afcaf * s.fcafp + Afcas * s.fcas;
         // nca (:613-617): anca*k2n/km2n relaxes with rate km2n = jca
         const R km2n = s.jca;
@@ -325,7 +341,7 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R q2 = q * q;
         const R anca = M::rcp(R(1000) * M::rcp(km2n) + q2 * q2);
         const R ninf = anca * R(1000) * M::rcp(km2n);
-        s.nca = ninf - (ninf - s.nca) * M::decay(ndt, km2n);
+        s.nca = M::rl_slow(s.nca, ninf, dt, km2n);
         const R nca = s.nca, jca = s.jca;
         const R PhiCaL = R(2.0 * k::F) * r2 * (cass * e2 - R(0.341 * k::cao));
         const R PhiCaNa = R(k::F) * r1 * (R(0.75) * nass * e1 - R(0.75 * k::nao));
@@ -349,8 +365,8 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R txrs = R(1.865) + M::rcp(R(0.06629) * M::exl(v, 1.0 / 7.355, -34.70 / 7.355) +
                                          R(1.128e-5) * M::exl(v, -1.0 / 25.94, 29.74 / 25.94));
         const R Axrf = M::rcp(R(1) + M::exl(v, 1.0 / 38.21, 54.81 / 38.21));
-        s.xrf = xrss - (xrss - s.xrf) * M::decay(ndt, M::rcp(txrf));
-        s.xrs = xrss - (xrss - s.xrs) * M::decay(ndt, M::rcp(txrs));
+        s.xrf = M::rl_slow(s.xrf, xrss, dt, M::rcp(txrf));
+        s.xrs = M::rl_slow(s.xrs, xrss, dt, M::rcp(txrs));
         const R xr = Axrf * s.xrf + (R(1) - Axrf) * s.xrs;
         const R rkr = M::rcp((R(1) + M::exl(v, 1.0 / 75.0, 55.0 / 75.0)) * (R(1) + M::exl(v, 1.0 / 30.0, -10.0 / 30.0)));
         const R GKr = epi ? R(0.046 * 1.3) : (mid ? R(0.046 * 0.8) : R(0.046));
@@ -363,7 +379,7 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         const R xs1ss = M::rcp(R(1) + M::exl(v, -1.0 / 8.932, -11.60 / 8.932));
         const R txs1 = R(817.3) + M::rcp(R(2.326e-4) * M::exl(v, 1.0 / 17.80, 48.28 / 17.80) +
                                          R(0.001292) * M::exl(v, -1.0 / 230.0, -210.0 / 230.0));
-        s.xs1 = xs1ss - (xs1ss - s.xs1) * M::decay(ndt, M::rcp(txs1));
+        s.xs1 = M::rl_slow(s.xs1, xs1ss, dt, M::rcp(txs1));
         const R rate_xs2 = R(0.01) * M::exl(v, 1.0 / 20.0, -50.0 / 20.0) + R(0.0193) * M::exl(v, -1.0 / 31.0, -66.54 / 31.0);
         s.xs2 = xs1ss - (xs1ss - s.xs2) * M::decay(ndt, rate_xs2);
         const R KsCa = R(1) + R(0.6) * M::rcp(R(1) + M::pow14(R(3.8e-5) * M::rcp(cai)));

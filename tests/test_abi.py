@@ -76,3 +76,22 @@ def test_product_does_not_import_the_oracle():
                 txt = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in txt and "from oracle" not in txt, f
                 assert "oracle.h" not in txt and "liboracle" not in txt, f
+
+
+def test_fp32_step_limit_is_enforced():
+    """FP32 handles refuse dt above LQT_FP32_MAX_DT (the slow gates' series
+    Rush-Larsen step is accurate to dt/tau = 0.125); FP64 takes it."""
+    lib = A.lib()
+    h = C.c_void_p()
+    p = A.preset()
+    ct = np.array([3], np.uint8)
+    st = lib.lqt_cells_create(1, L.ptr(ct), C.byref(p), 1, A.FP32, 0.2, C.byref(h))
+    assert st == L.LQT_ERR_ARG and b"FP32 needs dt" in lib.lqt_last_error()
+    with pytest.raises(A.LqtError, match="FP32 needs dt"):
+        A.Tissue(np.full((3, 3, 3), 3, np.uint8), None, 0.2, 0.154, 0.154, A.preset(), A.FP32, 0.2)
+    with pytest.raises(A.LqtError, match="FP32 needs dt"):
+        A.Strands([A.StrandSpec(A.transmural(10), A.preset())], A.FP32, 0.2)
+    st = lib.lqt_cells_create(1, L.ptr(ct), C.byref(p), 1, A.FP64, 0.2, C.byref(h))
+    assert st != L.LQT_ERR_ARG  # no device here: LQT_ERR_HIP; on a GPU: created
+    if st == L.LQT_OK:
+        lib.lqt_cells_destroy(h)
