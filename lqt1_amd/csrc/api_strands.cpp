@@ -45,8 +45,10 @@ struct lqt_strands {
     std::vector<uint8_t> label;  // per cell enum CellType
     std::vector<StrandDev> hdev;
     std::vector<Bundle> hbund;   // the bundles of all modes
+    std::vector<int64_t> s1r_h;  // S1 step ranges of all strands
     // device
     StrandDev* ddesc = nullptr;
+    int64_t* s1r = nullptr;
     Bundle* bund = nullptr;
     int* queue = nullptr;  // one work counter per layout entry
     uint8_t* cls = nullptr;
@@ -89,6 +91,7 @@ static void fill_sargs(lqt_strands* s, StrandsArgs<R>& a) {
     a.nbundles = 0;
     a.bundles = s->bund;
     a.desc = s->ddesc;
+    a.s1r = s->s1r;
     a.queue = s->queue;
     a.cls = s->cls;
     a.coeff = s->coeff;
@@ -206,7 +209,8 @@ static double qt_from_ecg(const T* phi, int64_t n, double t0, double dts, double
 // for every stop time.  The accumulator runs through the same values for
 // every strand, so it is run once up to the largest stop and each count is
 // a binary search in it.
-static std::vector<int64_t> step_counts(const std::vector<double>& stops, double dt) {
+static std::vector<int64_t> step_counts(const std::vector<double>& stops, double dt,
+                                        std::vector<double>* seq_out = nullptr) {
     double tmax = 0.0;
     for (double x : stops) tmax = std::max(tmax, x);
     std::vector<double> seq;  // time at the start of step k
@@ -218,7 +222,84 @@ static std::vector<int64_t> step_counts(const std::vector<double>& stops, double
     std::vector<int64_t> out(stops.size());
     for (size_t i = 0; i < stops.size(); ++i)
         out[i] = std::upper_bound(seq.begin(), seq.end(), stops[i]) - seq.begin();
+    if (seq_out) seq_out->swap(seq);
     return out;
+}
+
+// The loop's time tests (OneD.cc:337-347 and OneD_VW.cc:291-301), each in
+// its own statements so that no multiply-add is fused: the same roundings
+// as the reference's x86 build.
+static bool s1_window(double t, const StrandDev& d) {
+    const double fl = floor(t / d.bcl);
+    const double base = fl * d.bcl;
+    const double tc = t - base;
+    return tc >= d.stim_start && tc < d.stim_start + d.stim_dur;
+}
+
+// First step index in [0, n) whose time satisfies a predicate that is false
+// and then true along the accumulator (n if never).
+template <class P>
+static int64_t first_step(const std::vector<double>& seq, int64_t n, P pred) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (pred(seq[mid])) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// Step-index form of the strand's time tests over its nsteps steps (seq:
+// the accumulator's times, at least nsteps of them).  The S1 test is
+// periodic: each beat's window is searched near its nominal start, where
+// the in-cycle time rises monotonically, and the scan never goes back.
+static void strand_events(StrandDev& h, const std::vector<double>& seq, std::vector<int64_t>& s1r) {
+    const int64_t n = h.nsteps;
+    const double dt = n > 1 ? seq[1] - seq[0] : 0.02;
+    h.s1_first = (int64_t)s1r.size() / 2;
+    int64_t cnt = 0, kscan = 0;
+    if (n > 0 && h.stim_dur > 0) {
+        const double tlast = seq[n - 1];
+        const int64_t nb = (int64_t)floor(tlast / h.bcl) + 1;
+        for (int64_t b = 0; b <= nb; ++b) {
+            const double t_open = b * h.bcl + h.stim_start;
+            if (t_open - 4 * dt > tlast) break;
+            int64_t k = std::max<int64_t>(kscan, first_step(seq, n, [&](double t) { return t >= t_open - 4 * dt; }));
+            bool on = false;
+            int64_t k_on = 0;
+            for (; k < n && seq[k] < t_open + h.stim_dur + 4 * dt; ++k) {
+                const bool w = s1_window(seq[k], h);
+                if (w && !on) k_on = k;
+                if (!w && on) {
+                    s1r.push_back(k_on);
+                    s1r.push_back(k);
+                    ++cnt;
+                }
+                on = w;
+            }
+            if (on) {  // the window runs to the scan's end: extend it step by step
+                while (k < n && s1_window(seq[k], h)) ++k;
+                s1r.push_back(k_on);
+                s1r.push_back(k);
+                ++cnt;
+            }
+            kscan = k;
+        }
+    }
+    h.s1_count = cnt;
+    const double s2_end = h.s2_start + h.s2_dur;
+    h.k_wmax = first_step(seq, n, [&](double t) { return t - h.s2_base >= h.s2_start; });
+    h.k_s2a = h.s2_dur > 0 ? h.k_wmax : n;
+    h.k_s2b = h.s2_dur > 0 ? first_step(seq, n, [&](double t) { return !(t - h.s2_base < s2_end); }) : n;
+    h.k_mb0 = first_step(seq, n, [&](double t) { return floor(t / h.bcl) >= h.measure_beat; });
+    h.k_mb1 = first_step(seq, n, [&](double t) { return floor(t / h.bcl) >= h.measure_beat + 1.0; });
+    const int64_t k_rec = first_step(seq, n, [&](double t) { return t >= h.rec_from; });
+    auto next_multiple = [&](int64_t k, int every) -> int64_t {
+        if (every <= 0 || k >= n) return INT64_MAX;
+        return (k + every - 1) / every * every;
+    };
+    h.k_rec0 = next_multiple(k_rec, h.rec_every);
+    h.k_vrec0 = next_multiple(k_rec, h.vrec_every);
 }
 
 static int strand_waves(const StrandDev& d) { return (d.ncell + 63) / 64; }
@@ -391,8 +472,12 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
         {
             std::vector<double> stops(nstrands);
             for (int i = 0; i < nstrands; ++i) stops[i] = desc[i].stop_time;
-            const std::vector<int64_t> ns = step_counts(stops, dt);
-            for (int i = 0; i < nstrands; ++i) s->hdev[i].nsteps = ns[i];
+            std::vector<double> seq;
+            const std::vector<int64_t> ns = step_counts(stops, dt, &seq);
+            for (int i = 0; i < nstrands; ++i) {
+                s->hdev[i].nsteps = ns[i];
+                strand_events(s->hdev[i], seq, s->s1r_h);
+            }
         }
         // Layout 1: every strand at the largest block, most work first (step
         // count x wavefronts).  Layout 0: strands grouped by their own
@@ -448,6 +533,8 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
         for (int i = 0; i < nparams; ++i) v0[i] = params[i].v0;
         lqt_status e;
         if ((e = upload(&s->ddesc, s->hdev.data(), nstrands))) return e;
+        if (s->s1r_h.empty()) s->s1r_h.assign(2, 0);  // a valid pointer even with no S1
+        if ((e = upload(&s->s1r, s->s1r_h.data(), s->s1r_h.size()))) return e;
         if ((e = upload(&s->bund, s->hbund.data(), s->hbund.size()))) return e;  // all layouts' bundles
         if ((e = upload(&s->queue, (const int*)nullptr, 8))) return e;
         if ((e = upload(&s->cls, cl.data(), ncells))) return e;
@@ -499,6 +586,7 @@ lqt_status lqt_strands_destroy(lqt_strands* s) {
     }
     if (s->fork) hipEventDestroy(s->fork);
     hipFree(s->ddesc);
+    hipFree(s->s1r);
     hipFree(s->bund);
     hipFree(s->queue);
     hipFree(s->cls);

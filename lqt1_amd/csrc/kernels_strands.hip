@@ -98,10 +98,22 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
         int fin_slot = -1;     // slot whose per-wave partials wait for the lead lane
         int64_t fin_idx = 0;
         int64_t pend_idx = 0;
+        // the next S1 range and sample steps (host-evaluated decisions)
+        int64_t s1i = 0, s1_on = INT64_MAX, s1_off = INT64_MAX;
+        if (has && D.s1_count > 0) {
+            s1_on = A.s1r[2 * D.s1_first];
+            s1_off = A.s1r[2 * D.s1_first + 1];
+        }
+        int64_t next_rec = D.k_rec0, next_vrec = D.k_vrec0;
         double time = 0.0;
         int b = 0;
         for (int64_t k = 0; k < kmax; ++k, time += A.dt_d) {
             const bool run = has && k < D.nsteps;
+            if (k >= s1_off) {  // past this S1 range: load the next one
+                ++s1i;
+                s1_on = s1i < D.s1_count ? A.s1r[2 * (D.s1_first + s1i)] : INT64_MAX;
+                s1_off = s1i < D.s1_count ? A.s1r[2 * (D.s1_first + s1i) + 1] : INT64_MAX;
+            }
             // V^n into LDS with mirrored ends
             if (live) {
                 vbuf[b][p] = v;
@@ -132,19 +144,17 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
                     const R first = (j == 0 || j == n - 1) ? (vr - vl) * inv_dx : (vr - vl) * inv_2dx;
                     const R second = (vr + vl - R(2) * vc) * inv_dx2;
                     const R dvgap = RdD * first + RD * second;
-                    const double tc = time - floor(time / D.bcl) * D.bcl;
-                    const double t2 = time - D.s2_base;
                     R ist = R(0);
-                    if (tc >= D.stim_start && tc < D.stim_start + D.stim_dur) {
+                    if (k >= s1_on) {  // in-cycle time in the S1 window
                         if (j < D.nstim) ist = (R)D.stim_strength;
-                    } else if (t2 >= D.s2_start && t2 < D.s2_start + D.s2_dur && s2_cell) {
+                    } else if (k >= D.k_s2a && k < D.k_s2b && s2_cell) {
                         ist = (R)D.s2_strength;
                     }
                     v = tpord_update<R, false>(s, v, c, cc, A.dt, ist, dvgap, nullptr);
-                    if (t2 >= D.s2_start && (double)v >= wmax) wmax = (double)v;
+                    if (k >= D.k_wmax && (double)v >= wmax) wmax = (double)v;
 
                     // beat tracker, in the measured beat only (floor(t/BCL) == k)
-                    if (floor(time / D.bcl) == D.measure_beat) {
+                    if (k >= D.k_mb0 && k < D.k_mb1) {
                         const double vf = (double)v;
                         if (!(tr.flags & 1)) {
                             tr.flags |= 1;
@@ -171,15 +181,21 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
                         }
                     }
                 }
-                if (D.rec_every > 0 && (k % D.rec_every) == 0 && time >= D.rec_from && nrec < D.phi_cap) {
-                    if (nrec == 0 && lead) A.phit0[sidx] = time;
-                    pend = 1;
-                    pend_idx = nrec++;
+                if (k == next_rec) {  // k % rec_every == 0 and time >= rec_from
+                    next_rec += D.rec_every;
+                    if (nrec < D.phi_cap) {
+                        if (nrec == 0 && lead) A.phit0[sidx] = time;
+                        pend = 1;
+                        pend_idx = nrec++;
+                    }
                 }
-                if (D.vrec_every > 0 && (k % D.vrec_every) == 0 && time >= D.rec_from && nvrow < D.vrec_cap) {
-                    if (live) A.vrec[D.vrec_off + nvrow * (int64_t)n + j] = (double)v;
-                    if (nvrow == 0 && lead) A.vrect0[sidx] = time;
-                    ++nvrow;
+                if (k == next_vrec) {
+                    next_vrec += D.vrec_every;
+                    if (nvrow < D.vrec_cap) {
+                        if (live) A.vrec[D.vrec_off + nvrow * (int64_t)n + j] = (double)v;
+                        if (nvrow == 0 && lead) A.vrect0[sidx] = time;
+                        ++nvrow;
+                    }
                 }
             }
             b ^= 1;

@@ -17,6 +17,14 @@ struct StrandDev {
     // steps of `for (time = 0; time <= stop_time; time += dt)`, counted on
     // the host with the same double accumulator
     int64_t nsteps;
+    // The time-dependent decisions of the loop as step indices, evaluated on
+    // the host on the same accumulator with the loop's own expressions: S1
+    // is on in the ranges s1r[s1_first .. s1_first + s1_count) (pairs
+    // [on, off)); S2 window [k_s2a, k_s2b); end-cell peaks from k_wmax;
+    // measured beat [k_mb0, k_mb1); first pseudo-ECG and V-row samples
+    // k_rec0, k_vrec0 (then every rec_every / vrec_every steps).
+    int64_t s1_first, s1_count;
+    int64_t k_s2a, k_s2b, k_wmax, k_mb0, k_mb1, k_rec0, k_vrec0;
 };
 
 // Strands that share one workgroup: strand s occupies the whole wavefronts
@@ -48,6 +56,7 @@ struct StrandsArgs {
     const Bundle* bundles;     // this launch's bundles, processing order
     int* queue;                // work counter, zeroed before each run
     const StrandDev* desc;
+    const int64_t* s1r;        // S1 step ranges of all strands, pairs [on, off)
     const uint8_t* cls;        // per cell class
     const double* coeff;       // per cell D (mm^2/ms)
     const Consts<R>* consts;
