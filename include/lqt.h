@@ -207,8 +207,10 @@ lqt_status lqt_tissue_set_variant(lqt_tissue *t, int i);
 lqt_status lqt_tissue_get_variant(const lqt_tissue *t, int *i);
 /* Autotuner: runs every launch shape for trial_steps steps from the current
  * state (state, V, guard and clock are restored after each trial, so the
- * simulation is unchanged), keeps the fastest and returns it in *best;
- * us[count] (may be NULL) receives each shape's mean kernel time. */
+ * simulation is unchanged; activation trackers are held meanwhile), twice,
+ * in opposite orders after a warm-up, keeps the fastest and returns it in
+ * *best; us[count] (may be NULL) receives each shape's mean time per step
+ * over its two trials. */
 lqt_status lqt_tissue_autotune(lqt_tissue *t, int trial_steps, int *best, double *us);
 /* Average device time of the fused step kernel over the last call of
  * lqt_tissue_step, in microseconds (HIP events on the tissue's stream, on

@@ -731,23 +731,52 @@ lqt_status lqt_tissue_autotune(lqt_tissue* t, int trial_steps, int* best, double
         release();
         return set_error(LQT_ERR_HIP, "lqt_tissue_autotune: snapshot copy failed");
     }
-    double tbest = 1e300;
-    int ibest = keep;
+    // Each shape runs trial_steps steps from the snapshot, timed as a whole
+    // by one event pair; the shapes run twice, in opposite orders, after a
+    // warm-up, so that the clocks' ramp and drift do not favour any of them.
+    // The trackers are held off: the trials are not part of the simulation.
+    const bool trk = t->trk_on;
+    t->trk_on = false;
+    hipEvent_t ea = nullptr, eb = nullptr;
     lqt_status st = LQT_OK;
-    for (int v = 0; v < kNumTissueVariants && !st; ++v) {
-        t->variant = v;
-        st = lqt_tissue_step(t, 1);  // first launch of this shape, untimed
-        if (!st) st = lqt_tissue_step(t, trial_steps);
-        const double u = t->us_step;
-        if (us) us[v] = u;
-        if (!st && u < tbest) {
-            tbest = u;
-            ibest = v;
-        }
+    if (hipEventCreate(&ea) != hipSuccess || hipEventCreate(&eb) != hipSuccess)
+        st = set_error(LQT_ERR_HIP, "lqt_tissue_autotune: event creation failed");
+    std::vector<double> acc(kNumTissueVariants, 0.0);
+    auto restore = [&]() {
         t->time = time0;
         t->step = step0;
         t->cur = cur0;
         if (copy_all(false) != hipSuccess && !st) st = set_error(LQT_ERR_HIP, "lqt_tissue_autotune: restore failed");
+    };
+    if (!st) {
+        t->variant = keep;
+        st = lqt_tissue_step(t, trial_steps);  // warm-up
+        restore();
+    }
+    for (int pass = 0; pass < 2 && !st; ++pass) {
+        for (int i = 0; i < kNumTissueVariants && !st; ++i) {
+            const int v = pass == 0 ? i : kNumTissueVariants - 1 - i;
+            t->variant = v;
+            st = lqt_tissue_step(t, 1);  // first launch of this shape, untimed
+            if (!st && hipEventRecord(ea, t->stream) != hipSuccess) st = set_error(LQT_ERR_HIP, "event record failed");
+            if (!st) st = lqt_tissue_step_async(t, trial_steps, nullptr);
+            if (!st && (hipEventRecord(eb, t->stream) != hipSuccess || hipEventSynchronize(eb) != hipSuccess))
+                st = set_error(LQT_ERR_HIP, "lqt_tissue_autotune: timing failed");
+            float ms = 0.f;
+            if (!st && hipEventElapsedTime(&ms, ea, eb) != hipSuccess)
+                st = set_error(LQT_ERR_HIP, "lqt_tissue_autotune: timing failed");
+            acc[v] += 1000.0 * ms / trial_steps;
+            restore();
+        }
+    }
+    if (ea) hipEventDestroy(ea);
+    if (eb) hipEventDestroy(eb);
+    t->trk_on = trk;
+    int ibest = keep;
+    for (int v = 0; v < kNumTissueVariants; ++v) {
+        acc[v] *= 0.5;
+        if (us) us[v] = acc[v];
+        if (acc[v] < acc[ibest]) ibest = v;
     }
     release();
     t->variant = st ? keep : ibest;

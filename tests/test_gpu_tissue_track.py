@@ -6,6 +6,7 @@ of a small anisotropic shell, 450 ms.
 * FP32 against FP64 on identical inputs: activation within one step
   (0.02 ms) and APD90 within 1 ms on every voxel, the accuracy gate of the
   FP32 path.
+* The autotuner's trial steps leave the maps alone.
 """
 import numpy as np
 import pytest
@@ -113,3 +114,22 @@ def test_fp32_maps_within_gates(runs):
     print(f"FP32 vs FP64: max |d act| {d_act:.3f} ms, max |d APD90| {d_apd:.3f} ms over {a64.size} voxels")
     assert d_act <= DT + 1e-9
     assert d_apd <= 1.0
+
+
+def test_autotune_leaves_the_maps_alone(lqt):
+    """The autotuner's trial steps do not reach the trackers: maps of a run
+    tuned half-way equal those of the same run untuned."""
+    lab, fib, sites = _shell()
+    out = []
+    for tune in (False, True):
+        T = lqt.Tissue(lab, fib, DX, 0.154, 0.0385, lqt.preset("cpu_default"), lqt.FP32, DT)
+        T.set_stimulus(sites, None, bcl=1000.0, duration=1.0, strength=-80.0)
+        T.track(0.0, 30.0)
+        T.step(100)
+        if tune:
+            T.autotune(4)
+        T.step(1400)
+        out.append(T.track_results())
+        T.close()
+    assert np.all(out[0][0] > 0)
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
