@@ -81,7 +81,8 @@ lqt_status lqt_bench_copy(int64_t bytes, int reps, double *us_per_copy);
  */
 typedef struct lqt_cells lqt_cells;
 
-/* celltype[n] in enum CellType; params[nparams] with nparams 1 or n. */
+/* celltype[n] in enum CellType; params[nparams] with nparams 1 or n;
+ * n <= 2^29 - 64 (and at most 2^29 - 64 cells in all for a strand batch). */
 lqt_status lqt_cells_create(int n, const uint8_t *celltype, const lqt_params *params, int nparams,
                             lqt_precision prec, double dt, lqt_cells **out);
 lqt_status lqt_cells_destroy(lqt_cells *c);
@@ -141,6 +142,9 @@ typedef struct {
 
 typedef struct lqt_tissue lqt_tissue;
 
+/* The padded grid, (nx+2)(ny+2)(nz+2) cells, must stay below 2^30 cells in
+ * FP32 and 2^29 in FP64 (the step kernel addresses V by 32-bit byte
+ * offsets); the reference's 690 x 489 x 537 grid is 1.8e8. */
 lqt_status lqt_tissue_create(const lqt_geometry *geo, const lqt_params *params, lqt_precision prec,
                              double dt, lqt_tissue **out);
 lqt_status lqt_tissue_destroy(lqt_tissue *t);

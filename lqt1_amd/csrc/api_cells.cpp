@@ -204,6 +204,8 @@ lqt_status lqt_cells_create(int n, const uint8_t* celltype, const lqt_params* pa
                             lqt_precision prec, double dt, lqt_cells** out) {
     LQT_CHECK(n > 0 && celltype && params && out, LQT_ERR_ARG, "lqt_cells_create: bad argument");
     LQT_CHECK(nparams == 1 || nparams == n, LQT_ERR_ARG, "lqt_cells_create: nparams must be 1 or n");
+    // state slots are addressed by 32-bit byte offsets of up to 8 bytes per cell
+    LQT_CHECK(n <= (1 << 29) - 64, LQT_ERR_ARG, "lqt_cells_create: at most 2^29 - 64 cells per handle");
     LQT_CHECK(dt > 0, LQT_ERR_ARG, "lqt_cells_create: dt must be > 0");
     LQT_CHECK(prec == LQT_FP64 || dt <= LQT_FP32_MAX_DT, LQT_ERR_ARG, "lqt_cells_create: FP32 needs dt <= %g ms",
               LQT_FP32_MAX_DT);

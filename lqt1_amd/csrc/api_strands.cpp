@@ -409,6 +409,8 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
         ncells = std::max<int64_t>(ncells, (int64_t)d.cell_offset + d.ncell);
         maxl = std::max(maxl, d.ncell);
     }
+    // the final-state slots are addressed by 32-bit byte offsets
+    LQT_CHECK(ncells <= (1 << 29) - 64, LQT_ERR_ARG, "lqt_strands_create: at most 2^29 - 64 cells in all");
     lqt_strands* s = new lqt_strands();
     s->n = nstrands;
     s->ncells = ncells;

@@ -274,6 +274,10 @@ lqt_status lqt_tissue_create(const lqt_geometry* geo, const lqt_params* params, 
     const int64_t ncell = (int64_t)geo->nx * geo->ny * geo->nz;
     const int64_t npad = (int64_t)(geo->nx + 2) * (geo->ny + 2) * (geo->nz + 2);
     LQT_CHECK(npad < (int64_t)INT32_MAX, LQT_ERR_ARG, "lqt_tissue_create: grid too large for 32-bit indices");
+    // the step kernel addresses the V grid by 32-bit byte offsets
+    LQT_CHECK(npad * (prec == LQT_FP32 ? 4 : 8) < ((int64_t)1 << 32), LQT_ERR_ARG,
+              "lqt_tissue_create: padded grid of %lld cells exceeds the %s limit of 2^%d", (long long)npad,
+              prec == LQT_FP32 ? "FP32" : "FP64", prec == LQT_FP32 ? 30 : 29);
     int ndev = 0;
     LQT_CHECK(hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0, LQT_ERR_HIP,
               "lqt_tissue_create: no HIP device available");

@@ -95,3 +95,17 @@ def test_fp32_step_limit_is_enforced():
     assert st != L.LQT_ERR_ARG  # no device here: LQT_ERR_HIP; on a GPU: created
     if st == L.LQT_OK:
         lib.lqt_cells_destroy(h)
+
+
+def test_sizes_beyond_32_bit_offsets_are_refused():
+    """The kernels address the V grid and the state slots by 32-bit byte
+    offsets; handles that would overflow them are refused, not run."""
+    lib = A.lib()
+    h = C.c_void_p()
+    p = A.preset()
+    ct = np.array([3], np.uint8)
+    st = lib.lqt_cells_create(1 << 29, L.ptr(ct), C.byref(p), 1, A.FP64, 0.02, C.byref(h))
+    assert st == L.LQT_ERR_ARG and b"2^29" in lib.lqt_last_error()
+    big = np.zeros((1024, 1024, 520), np.uint8)  # 549M padded cells: over 2^29 in FP64
+    with pytest.raises(A.LqtError, match="limit of 2"):
+        A.Tissue(big, None, 0.2, 0.154, 0.154, A.preset(), A.FP64, 0.02)
