@@ -2,9 +2,11 @@
 // strands at a time from a work counter (each strand on whole wavefronts,
 // cells on lanes, state in registers for the whole run) and advances every
 // strand of the bundle with the loop of OneD.cc:315-436: Jacobi snapshot of
-// V in LDS, no-flux ends by copy, S1 on the first cells by the in-cycle time
-// of the double time accumulator (or, while that window is closed, the S2 of
-// OneD_VW.cc:291-301), then the TPORd update.  The pseudo-ECG of
+// V in LDS, no-flux ends by copy, S1 on the first cells while the in-cycle
+// time of the double time accumulator is in its window (or, while that
+// window is closed, the S2 of OneD_VW.cc:291-301), then the TPORd update.
+// The time tests arrive as step indices the host evaluated on the same
+// accumulator (StrandDev), so a step compares integers.  The pseudo-ECG of
 // PlotScript/plot1DECG_compare.py:89-105 is reduced across each strand every
 // rec_every steps (wavefront shuffles, then one LDS slot per wavefront,
 // summed by the strand's first lane one step later).
