@@ -8,6 +8,8 @@
   FP64 oracle of OneD.cc, in the per-size and the packed layouts.
 * Empty inputs, strands outside the documented 2..512 cells and a
   pseudo-ECG electrode on a cell are refused with an error, not run.
+* Strands whose BCL and stimulus times are not multiples of dt step as the
+  FP64 oracle of OneD.cc's loop (the kernel's host-evaluated time tests).
 """
 import numpy as np
 import pytest
@@ -96,3 +98,26 @@ def test_empty_inputs_are_refused(lqt):
     with pytest.raises(lqt.LqtError, match="electrode"):
         lqt.Strands([lqt.StrandSpec(lqt.transmural(n), lqt.preset("cpu_default"), nbeats=1,
                                     electrode_mm=-(n * 0.15))])
+
+
+def test_strand_time_tests_with_odd_protocol_match_oracle(lqt, oracle):
+    """The loop's time tests arrive at the kernel as step indices evaluated on
+    the host; with a BCL, stimulus start and duration that are not multiples
+    of dt (and a window opening at the cycle's start, where floor(t/BCL)
+    switches) the steps they select are still those of the reference's
+    loop: FP64 strands equal the oracle run of OneD.cc's loop."""
+    O = oracle
+    cases = [dict(bcl=333.3, stim_start=12.345, stim_duration=1.111),
+             dict(bcl=400.0, stim_start=0.0, stim_duration=2.345)]
+    specs = [lqt.StrandSpec(lqt.transmural(24), lqt.preset("cpu_default"), nbeats=3, nstim=2, **c) for c in cases]
+    st = lqt.Strands(specs, lqt.FP64, 0.02)
+    st.run()
+    res = st.results()
+    for i, (spec, c) in enumerate(zip(specs, cases)):
+        r = _oracle(O, spec)
+        act, _, _ = st.cell_results(i)
+        assert res[i]["steps"] == r["steps"], c
+        assert np.all(act > 0) and np.max(np.abs(act - r["act"])) < 1e-6, c
+        fin = st.final_state(i)
+        assert np.max(np.abs(fin[:, 0] - r["state"][:, 0])) < 1e-6, c
+    st.close()
