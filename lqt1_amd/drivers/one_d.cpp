@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/lqt.h"
+#include "outdir.hpp"
 
 static int die(const char* what) {
     fprintf(stderr, "%s: %s\n", what, lqt_last_error());
@@ -77,6 +78,10 @@ int main(int argc, char** argv) {
     lqt_strands_vtrace(s, 0, nullptr, &t0, &rows);
     std::vector<double> v((size_t)rows * n);
     if (lqt_strands_vtrace(s, 0, v.data(), &t0, &rows)) return die("lqt_strands_vtrace");
+    if (!make_dirs(parent_dir(out))) {
+        fprintf(stderr, "cannot create the folder of %s\n", out.c_str());
+        return 1;
+    }
     FILE* f = fopen(out.c_str(), "w+");
     if (!f) {
         fprintf(stderr, "cannot open %s\n", out.c_str());

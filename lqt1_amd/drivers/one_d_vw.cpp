@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/lqt.h"
+#include "outdir.hpp"
 
 static int die(const char* what) {
     fprintf(stderr, "%s: %s\n", what, lqt_last_error());
@@ -98,6 +99,10 @@ int main(int argc, char** argv) {
     fprintf(stderr, "%lld candidates visited, %lld strands in %d launches, %.1f ms on the device\n",
             (long long)r.visited, (long long)r.simulated, r.launches, r.device_ms);
 
+    if (files && r.visited > 0 && !make_dirs(outdir)) {
+        fprintf(stderr, "cannot create %s\n", outdir.c_str());
+        return 1;
+    }
     if (files && r.visited > 0) {
         // rows of the last beat every 50 steps (OneD_VW.cc:370-385): replay
         // the double time accumulator to find the recorded steps

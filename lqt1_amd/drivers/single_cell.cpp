@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/lqt.h"
+#include "outdir.hpp"
 
 static int die(const char* what) {
     fprintf(stderr, "%s: %s\n", what, lqt_last_error());
@@ -55,6 +56,10 @@ int main(int argc, char** argv) {
     double stats[10];
     if (lqt_cells_pace(c, &pace, stats, 0, 1, trace ? tr.data() : nullptr, trace ? nsteps : 0, &ntr))
         return die("lqt_cells_pace");
+    if (!make_dirs(outdir)) {
+        fprintf(stderr, "cannot create %s\n", outdir.c_str());
+        return 1;
+    }
     if (trace) {
         std::string f = outdir + "/VentriSingleCellResults_LQT1_ISO_MCELL.dat";
         FILE* df = fopen(f.c_str(), "w+");
@@ -72,6 +77,10 @@ int main(int argc, char** argv) {
                              "APD50 = %.5f ms\n", "APD75 = %.5f ms\n", "APD80 = %.5f ms\n", "APD90 = %.5f ms\n"};
     std::string f = outdir + "/VentriSingleCellStats.dat";
     FILE* af = fopen(f.c_str(), "w+");
+    if (!af) {
+        fprintf(stderr, "cannot open %s\n", f.c_str());
+        return 1;
+    }
     for (int j = 0; j < 10; ++j) {
         if (af) fprintf(af, names[j], stats[j]);
         printf(names[j], stats[j]);

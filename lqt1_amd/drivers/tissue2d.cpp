@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/lqt.h"
+#include "outdir.hpp"
 
 static int die(const char* what) {
     fprintf(stderr, "%s: %s\n", what, lqt_last_error());
@@ -170,6 +171,10 @@ int main(int argc, char** argv) {
     while (time <= tmax) {
         if (per_vtk > 0 && count % per_vtk == 0) {
             printf("Progress = %g%%\n", 100.0 * time / tmax);
+            if (!make_dirs(outdir)) {
+                fprintf(stderr, "cannot create %s\n", outdir.c_str());
+                return 1;
+            }
             const std::string fn = outdir + "/ap_" + std::to_string((long long)(count / per_ms)) + ".vtk";
             if (lqt_tissue_write_vtk(t, fn.c_str(), 1)) return die("vtk");
             int64_t gd[3];

@@ -106,7 +106,7 @@ def test_config2_annulus(lqt, oracle):
 
 def test_oned_driver_writes_reference_file(lqt, oracle, tmp_path):
     exe = os.path.join(ROOT, "lqt1_amd", "bin", "lqt_oned")
-    out = tmp_path / "oned.dat"
+    out = tmp_path / "Outputs" / "oned.dat"  # the driver creates the missing folder
     r = subprocess.run([exe, "--numS1", "1", "--stop", "1200", "--out", str(out)], capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0, r.stderr
@@ -122,13 +122,14 @@ def test_oned_driver_writes_reference_file(lqt, oracle, tmp_path):
 
 def test_single_cell_driver_stats(lqt, oracle, tmp_path):
     exe = os.path.join(ROOT, "lqt1_amd", "bin", "lqt_single_cell")
-    r = subprocess.run([exe, "--beats", "3", "--outdir", str(tmp_path)], capture_output=True, text=True, timeout=300)
+    od = tmp_path / "Outputs" / "run1"  # created by the driver
+    r = subprocess.run([exe, "--beats", "3", "--outdir", str(od)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     st, _ = oracle.single_cell(oracle.MCELL, oracle.params(), nbeats=3)
-    txt = (tmp_path / "VentriSingleCellStats.dat").read_text()
+    txt = (od / "VentriSingleCellStats.dat").read_text()
     apd90 = float([ln for ln in txt.splitlines() if ln.startswith("APD90")][0].split("=")[1].split()[0])
     assert abs(apd90 - st["apd90"]) < 1e-4
-    rows = np.loadtxt(tmp_path / "VentriSingleCellResults_LQT1_ISO_MCELL.dat")
+    rows = np.loadtxt(od / "VentriSingleCellResults_LQT1_ISO_MCELL.dat")
     # one row per step of `for (time = 0; time < 3000; time += dt)`
     n = 0
     t = 0.0
