@@ -7,6 +7,8 @@ of a small anisotropic shell, 450 ms.
   (0.02 ms) and APD90 within 1 ms on every voxel, the accuracy gate of the
   FP32 path.
 * The autotuner's trial steps leave the maps alone.
+* The 2D annulus of the sheet benchmark: FP32 within the same gates over
+  its first beat, on every one of its 97.7k voxels.
 """
 import numpy as np
 import pytest
@@ -133,3 +135,28 @@ def test_autotune_leaves_the_maps_alone(lqt):
         T.close()
     assert np.all(out[0][0] > 0)
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_annulus_fp32_maps_within_gates(lqt):
+    """The 2D annulus of the sheet benchmark (97.7k voxels, isotropic, S1 from
+    the endo rim, ISO off): over its first beat every voxel activates within
+    one step of FP64 and repolarises within 1 ms of it in FP32."""
+    from lqt1_amd import geometry as G
+
+    geo = G.annulus_2d(seed=0)
+    maps = []
+    for prec in (lqt.FP64, lqt.FP32):
+        T = lqt.Tissue(geo.label, None, geo.dx, 0.154, 0.154, lqt.preset("config1"), prec, DT)
+        T.set_stimulus(geo.stim_sites, None, bcl=1000.0, duration=3.0, strength=-52.0)
+        T.track(0.0, 1000.0)
+        T.step(50001)
+        maps.append(T.track_results())
+        T.close()
+    (a64, p64), (a32, p32) = maps
+    assert np.all(a64 > 0) and np.all(a32 > 0)
+    assert np.all(p64 > 0) and np.all(p32 > 0), "every voxel repolarises within the beat"
+    d_act, d_apd = np.max(np.abs(a32 - a64)), np.max(np.abs(p32 - p64))
+    print(f"annulus FP32 vs FP64 over {a64.size} voxels: activation {a64.min():.2f}-{a64.max():.2f} ms, "
+          f"max |d act| {d_act:.3f} ms, APD90 {p64.min():.1f}-{p64.max():.1f} ms, max |d APD90| {d_apd:.3f} ms")
+    assert d_act <= DT + 1e-9
+    assert d_apd <= 1.0
