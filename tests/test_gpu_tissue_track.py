@@ -94,6 +94,12 @@ def runs(lqt, oracle):
         vs.append(s[:, 0].copy())
         t += DT
     ref = _track(times, vs)
+    # the vectorised rule above is the oracle's beat tracker (the C
+    # restatement pinned on synthetic waveforms): same answers on a sample
+    V = np.array(vs)
+    for k in np.linspace(0, nvox - 1, 24).astype(int):
+        a, d, _ = O.track_trace(times, V[:, k])
+        assert (a, d) == (ref[0][k], ref[1][k]), k
     g64 = _gpu(lqt, lqt.FP64, lab, fib, sites, len(times))
     g32 = _gpu(lqt, lqt.FP32, lab, fib, sites, len(times))
     return ref, g64, g32
