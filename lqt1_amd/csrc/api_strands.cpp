@@ -262,7 +262,8 @@ static void strand_events(StrandDev& h, const std::vector<double>& seq, std::vec
         const double tlast = seq[n - 1];
         const int64_t nb = (int64_t)floor(tlast / h.bcl) + 1;
         for (int64_t b = 0; b <= nb; ++b) {
-            const double t_open = b * h.bcl + h.stim_start;
+            // the window opens at in-cycle time max(stim_start, 0): tc >= 0
+            const double t_open = b * h.bcl + std::max(0.0, h.stim_start);
             if (t_open - 4 * dt > tlast) break;
             int64_t k = std::max<int64_t>(kscan, first_step(seq, n, [&](double t) { return t >= t_open - 4 * dt; }));
             bool on = false;

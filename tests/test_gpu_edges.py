@@ -103,12 +103,14 @@ def test_empty_inputs_are_refused(lqt):
 def test_strand_time_tests_with_odd_protocol_match_oracle(lqt, oracle):
     """The loop's time tests arrive at the kernel as step indices evaluated on
     the host; with a BCL, stimulus start and duration that are not multiples
-    of dt (and a window opening at the cycle's start, where floor(t/BCL)
-    switches) the steps they select are still those of the reference's
-    loop: FP64 strands equal the oracle run of OneD.cc's loop."""
+    of dt (and windows opening at the cycle's start, where floor(t/BCL)
+    switches, one of them from a negative start) the steps they select are
+    still those of the reference's loop: FP64 strands equal the oracle run
+    of OneD.cc's loop."""
     O = oracle
     cases = [dict(bcl=333.3, stim_start=12.345, stim_duration=1.111),
-             dict(bcl=400.0, stim_start=0.0, stim_duration=2.345)]
+             dict(bcl=400.0, stim_start=0.0, stim_duration=2.345),
+             dict(bcl=400.0, stim_start=-1.0, stim_duration=3.0)]
     specs = [lqt.StrandSpec(lqt.transmural(24), lqt.preset("cpu_default"), nbeats=3, nstim=2, **c) for c in cases]
     st = lqt.Strands(specs, lqt.FP64, 0.02)
     st.run()
