@@ -187,7 +187,8 @@ lqt_status lqt_tissue_v_guard(lqt_tissue *t, int64_t counts[3]);
  * later step whose V <= Vmax - 0.9 (Vmax - Vrest), with Vmax the peak since
  * activation and Vrest the V after the window's first step.  No counterpart
  * in the reference's tissue programs, which write V maps only.  Calling
- * lqt_tissue_track resets the trackers; t_to <= t_from switches them off. */
+ * lqt_tissue_track resets the trackers; t_to <= t_from switches them off.
+ * The trackers are not part of a checkpoint (lqt_tissue_save/load). */
 lqt_status lqt_tissue_track(lqt_tissue *t, double t_from, double t_to);
 /* act[nvox], apd90[nvox] in ms (either may be NULL), -1 where the voxel has
  * not activated / not repolarised within the window. */
