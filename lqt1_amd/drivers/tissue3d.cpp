@@ -20,6 +20,7 @@
 //   lqt_tissue3d --geometry FILE --dims X Y Z [--stim FILE] [--dx MM]
 //                [--time MS] [--bcl MS] [--preset NAME] [--fp64]
 //                [--dfibre D] [--dcross D] [--outdir DIR] [--vtk-every MS]
+//                [--maps FILE]  (activation time and APD90 per voxel: "x y z act apd90")
 //                [--vtk-stride N] [--island X Y Z R] [--s2-box X0 X1 Y0 Y1 Z0 Z1]
 //                [--s2-time MS] [--window MS] [--labels-out FILE]
 #include <math.h>
@@ -45,7 +46,7 @@ static uint8_t celltype_of(int l) {
 }
 
 int main(int argc, char** argv) {
-    std::string geo, stim, preset = "gpu3d", outdir = "Output3D", labels_out;
+    std::string geo, stim, preset = "gpu3d", outdir = "Output3D", labels_out, maps;
     int X = 690, Y = 489, Z = 537;  // GPU3D.cu:28-30
     double dx = 0.2, dt = 0.02, tmax = 60000.0, bcl = 500.0, vtk_every = 10.0;  // :41-42, :1725-1728
     double dfib = 0.154, dcross = 0.154;  // GPU3D.cu:37-38 (isotropic)
@@ -80,6 +81,7 @@ int main(int argc, char** argv) {
         } else if (a == "--s2-time") s2_time = atof(next());
         else if (a == "--window") window = atof(next());
         else if (a == "--labels-out") labels_out = next();
+        else if (a == "--maps") maps = next();
         else {
             fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
@@ -185,6 +187,8 @@ int main(int argc, char** argv) {
                                 s2cells.data()))
         return die("lqt_tissue_set_stimulus");
 
+    if (!maps.empty() && lqt_tissue_track(t, 0.0, 1e300)) return die("lqt_tissue_track");
+
     const int64_t per_vtk = (int64_t)(vtk_every / dt + 0.5);
     const int64_t per_ms = (int64_t)(1.0 / dt + 0.5);
     double time = 0.0;
@@ -214,6 +218,32 @@ int main(int argc, char** argv) {
         if (lqt_tissue_step(t, k)) return die("lqt_tissue_step");
         time = tt;
         step += k;
+    }
+    if (!maps.empty()) {
+        // activation time and APD90 of every tissue voxel, in the compacted
+        // (x, y, z) order of the label grid
+        int64_t nvox = 0, nghost = 0;
+        if (lqt_tissue_info(t, &nvox, &nghost)) return die("lqt_tissue_info");
+        std::vector<double> act((size_t)nvox), apd((size_t)nvox);
+        if (lqt_tissue_track_results(t, act.data(), apd.data())) return die("lqt_tissue_track_results");
+        if (!make_dirs(parent_dir(maps))) {
+            fprintf(stderr, "cannot create the folder of %s\n", maps.c_str());
+            return 1;
+        }
+        FILE* o = fopen(maps.c_str(), "w");
+        if (!o) {
+            fprintf(stderr, "cannot open %s\n", maps.c_str());
+            return 1;
+        }
+        int64_t k = 0;
+        for (int x = 0; x < X; ++x)
+            for (int y = 0; y < Y; ++y)
+                for (int z = 0; z < Z; ++z)
+                    if (label[((int64_t)x * Y + y) * Z + z] != LQT_NONE) {
+                        fprintf(o, "%d %d %d %.2f %.2f\n", x, y, z, act[k], apd[k]);
+                        ++k;
+                    }
+        fclose(o);
     }
     lqt_tissue_destroy(t);
     printf("All done.\n");

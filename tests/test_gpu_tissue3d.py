@@ -12,7 +12,8 @@ start at 81 ms (cut to 1 ms by the 82 ms window of GPU3D.cu:1757) and at
   stride 1) against the oracle stepping the same tissue with the same
   stimulus rule, to the 6 digits the file keeps; at 83 ms the same oracle
   without the 82 ms window differs by about 11 mV, so the comparison sees
-  the window.
+  the window;
+* the activation map the program writes with --maps against the oracle's.
 """
 import os
 import subprocess
@@ -105,6 +106,7 @@ def _oracle(O, tissue, ct, fib, rim, tact, snaps, window):
     s = np.tile(O.cell_init(-86.2), (nvox, 1))
     t = 0.0
     out = {}
+    act = np.full(nvox, -1.0)  # first step whose new V >= -30 mV (OneD.cc:422), at its start time
     for n in range(1, max(snaps) + 1):
         tc = t - np.floor(t / 500.0) * 500.0
         ist = np.zeros(nvox)
@@ -112,9 +114,12 @@ def _oracle(O, tissue, ct, fib, rim, tact, snaps, window):
             on = (tc >= off) & (tc < off + 3.0)
             ist[site[on]] = -52.0
         O.tissue_step(g, nvox, dten, drift, 0.2, 0.02, ct, p, s, ist)
+        up = (act < 0) & (s[:, 0] >= -30.0)
+        act[up] = t
         t += 0.02
         if n in snaps:
             out[n] = s[:, 0].copy()
+    out["act"] = act
     return out, cells
 
 
@@ -127,7 +132,8 @@ def test_ventricle_program_matches_oracle(lqt, oracle, tmp_path):
     r = subprocess.run([exe, "--geometry", str(tmp_path / "geo.txt"), "--dims", str(X), str(Y), str(Z),
                         "--stim", str(tmp_path / "stim.cnd"), "--time", "90.5", "--fp64", "--dfibre", "0.154",
                         "--dcross", "0.0385", "--vtk-every", "1", "--vtk-stride", "1", "--outdir", str(out),
-                        "--island", *[str(v) for v in ISLAND], "--labels-out", str(tmp_path / "lab.txt")],
+                        "--island", *[str(v) for v in ISLAND], "--labels-out", str(tmp_path / "lab.txt"),
+                        "--maps", str(tmp_path / "maps" / "act.txt")],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     assert f"There are {rim.shape[0]} stimulus points." in r.stdout
@@ -161,6 +167,17 @@ def test_ventricle_program_matches_oracle(lqt, oracle, tmp_path):
         print(f"3D ventricle program FP64 vs oracle at {ms} ms: max rel |dV| {err:.3g} over {cells.size} voxels")
         assert err < 1e-5, ms
         assert np.all(v[~tissue] == -100.0)
+    # the activation map the program writes (--maps) against the oracle's
+    # (times printed to 0.01 ms; the oracle ran 4500 of the program's 4526 steps)
+    mp = np.loadtxt(tmp_path / "maps" / "act.txt")
+    assert mp.shape == (cells.size, 5)
+    assert np.array_equal(mp[:, :3].astype(int), np.argwhere(tissue))
+    ao = ref["act"]
+    hit = ao >= 0
+    assert hit.sum() > 0.5 * cells.size
+    assert np.max(np.abs(mp[hit, 3] - ao[hit])) < 0.006
+    assert np.all((mp[~hit, 3] < 0) | (mp[~hit, 3] >= 89.99))
+    print(f"activation map: {hit.sum()} of {cells.size} voxels by 90 ms, equal to the oracle's")
     d_win = np.max(np.abs(nowin[4150] - ref[4150]))
     print(f"the 82 ms window changes V at 83 ms by up to {d_win:.1f} mV")
     assert d_win > 5.0
