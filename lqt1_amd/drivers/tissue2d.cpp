@@ -15,6 +15,7 @@
 //   lqt_tissue2d [--geometry FILE] [--stim FILE] [--dims X Y] [--dx MM]
 //                [--time MS] [--bcl MS] [--preset NAME] [--fp64]
 //                [--dfibre D] [--dcross D] [--outdir DIR] [--vtk-every MS]
+//                [--maps FILE]  (activation time and APD90 per cell: "x y act apd90")
 //                [--layer Z] [--island X0 X1 Y0 Y1] [--s2-time MS]
 //                [--labels-out FILE]
 #include <stdio.h>
@@ -41,7 +42,7 @@ static uint8_t celltype_of(int l) {
 
 int main(int argc, char** argv) {
     std::string geo = "157layerwithfibervector.txt", stim = "stimulationprofile.cnd", preset = "gpu2d",
-                outdir = "Outputs2Dreal", labels_out;
+                outdir = "Outputs2Dreal", labels_out, maps;
     int X = 690, Y = 489, layer = 157;       // new_11.cu:29-30, :1734
     double dx = 0.2, dt = 0.02;              // new_11.cu:62-63
     double tmax = 30000.0, bcl = 1000.0, vtk_every = 10.0;  // :1751-1755
@@ -72,6 +73,7 @@ int main(int argc, char** argv) {
             for (int k = 0; k < 4; ++k) isl[k] = atoi(next());
         } else if (a == "--s2-time") s2_time = atof(next());
         else if (a == "--labels-out") labels_out = next();
+        else if (a == "--maps") maps = next();
         else {
             fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
@@ -164,6 +166,8 @@ int main(int argc, char** argv) {
                                 s2cells.data()))
         return die("lqt_tissue_set_stimulus");
 
+    if (!maps.empty() && lqt_tissue_track(t, 0.0, 1e300)) return die("lqt_tissue_track");
+
     const int64_t per_vtk = (int64_t)(vtk_every / dt);
     const int64_t per_ms = (int64_t)(1 / dt);
     double time = 0.0;
@@ -192,6 +196,31 @@ int main(int argc, char** argv) {
         if (lqt_tissue_step(t, k)) return die("lqt_tissue_step");
         time = tt;
         count += k;
+    }
+    if (!maps.empty()) {
+        // activation time and APD90 of every tissue cell, in the compacted
+        // (x, y) order of the label grid
+        int64_t nvox = 0, nghost = 0;
+        if (lqt_tissue_info(t, &nvox, &nghost)) return die("lqt_tissue_info");
+        std::vector<double> act((size_t)nvox), apd((size_t)nvox);
+        if (lqt_tissue_track_results(t, act.data(), apd.data())) return die("lqt_tissue_track_results");
+        if (!make_dirs(parent_dir(maps))) {
+            fprintf(stderr, "cannot create the folder of %s\n", maps.c_str());
+            return 1;
+        }
+        FILE* o = fopen(maps.c_str(), "w");
+        if (!o) {
+            fprintf(stderr, "cannot open %s\n", maps.c_str());
+            return 1;
+        }
+        int64_t k = 0;
+        for (int xx = 0; xx < X; ++xx)
+            for (int yy = 0; yy < Y; ++yy)
+                if (label[(int64_t)xx * Y + yy] != LQT_NONE) {
+                    fprintf(o, "%d %d %.2f %.2f\n", xx, yy, act[k], apd[k]);
+                    ++k;
+                }
+        fclose(o);
     }
     lqt_tissue_destroy(t);
     printf("All done.\n");

@@ -8,7 +8,8 @@ rectangle) and a stimulation profile with rows on and off layer 157:
 * the label rules of new_11.cu:1460-1489 on hand-picked points;
 * FP64 V of the whole slice after 10 ms (the ap_10.vtk file) against the
   oracle stepping the same 2D sheet (9-point stencil, 2D ghost rule,
-  type-0 voxels held at -100 mV) to the 6 digits the file keeps.
+  type-0 voxels held at -100 mV) to the 6 digits the file keeps;
+* the activation map the program writes with --maps against the oracle's.
 """
 import os
 import subprocess
@@ -85,7 +86,8 @@ def test_slice_program_matches_oracle(lqt, oracle, tmp_path):
     out.mkdir()
     exe = os.path.join(ROOT, "lqt1_amd", "bin", "lqt_tissue2d")
     r = subprocess.run([exe, "--geometry", str(tmp_path / "slice.txt"), "--stim", str(tmp_path / "stim.cnd"),
-                        "--time", "10.5", "--fp64", "--outdir", str(out), "--labels-out", str(tmp_path / "lab.txt")],
+                        "--time", "10.5", "--fp64", "--outdir", str(out), "--labels-out", str(tmp_path / "lab.txt"),
+                        "--maps", str(tmp_path / "act.txt")],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     assert f"There are {rim.shape[0]} stimulus points." in r.stdout
@@ -123,11 +125,14 @@ def test_slice_program_matches_oracle(lqt, oracle, tmp_path):
     k = g.ravel()[rim[:, 0] * Y + rim[:, 1]] - 1
     p = O.params(iks_scale=0.05, iso=0)
     t = 0.0
+    act = np.full(nvox, -1.0)  # first step whose new V >= -30 mV, at its start time
     for _ in range(500):
         ist = np.zeros(nvox)
         if t - np.floor(t / 1000.0) * 1000.0 < 3.0:
             ist[k] = -52.0
         O.tissue_step(g, nvox, dten, drift, 0.2, 0.02, ct, p, s, ist)
+        up = (act < 0) & (s[:, 0] >= -30.0)
+        act[up] = t
         t += 0.02
     v = _read_vtk(out / "ap_10.vtk")
     vg = v.ravel()[cells]
@@ -138,3 +143,10 @@ def test_slice_program_matches_oracle(lqt, oracle, tmp_path):
     assert err < 1e-5
     assert np.all(v[~tissue] == -100.0)
     assert np.all(vg[ct >= 0x80] == -100.0)
+    # the activation map the program writes (--maps), printed to 0.01 ms; the
+    # program ran 526 steps to the oracle's 500
+    mp = np.loadtxt(tmp_path / "act.txt")
+    assert np.array_equal(mp[:, :2].astype(int), np.argwhere(tissue))
+    hit = act >= 0
+    assert hit.sum() > 100 and np.max(np.abs(mp[hit, 2] - act[hit])) < 0.006
+    assert np.all((mp[~hit, 2] < 0) | (mp[~hit, 2] >= 9.99))
