@@ -102,20 +102,25 @@ def _shell_depth(P, a_in, c_in, a_out, c_out, iters=16):
 
 def ventricle_3d(shape=(256, 256, 320), dx=0.2, seed=0, lv=(10.5, 44.0, 20.5, 54.0),
                  rv_wall=4.0, base_mm=58.0, island_mm=5.0, split=(0.30, 0.35, 0.35),
-                 fibre_epi=-60.0, fibre_endo=60.0, apex_stim_mm=4.0):
+                 fibre_epi=-60.0, fibre_endo=60.0, apex_stim_mm=4.0, scale=1.0):
     """Biventricular shell: LV truncated ellipsoid (inner a, c; outer a, c in
     mm), RV crescent of wall `rv_wall` on the +x side, base plane at z =
     base_mm above the apex.  Fibre angle goes linearly from fibre_epi at the
     epicardium to fibre_endo at the endocardium.  A sphere of M cells of
     radius island_mm sits mid-wall at a seed-chosen site.  Stimulus: LV
-    endocardial voxels within apex_stim_mm of the LV cavity apex."""
+    endocardial voxels within apex_stim_mm of the LV cavity apex.  `scale`
+    multiplies every length (a smaller heart on the same dx)."""
     rng = np.random.default_rng(seed)
     nx, ny, nz = shape
+    if scale != 1.0:
+        lv = tuple(scale * v for v in lv)
+        rv_wall, base_mm, island_mm, apex_stim_mm = (scale * rv_wall, scale * base_mm, scale * island_mm,
+                                                     scale * apex_stim_mm)
     a_in, c_in, a_out, c_out = lv
     # apex of the outer LV at z = 2 mm; long axis along +z
-    cx = (nx - 1) * dx / 2.0 - 4.0
+    cx = (nx - 1) * dx / 2.0 - 4.0 * scale
     cy = (ny - 1) * dx / 2.0
-    cz = 2.0 + c_out
+    cz = 2.0 * scale + c_out
     x = (np.arange(nx) * dx - cx).astype(np.float32)
     y = (np.arange(ny) * dx - cy).astype(np.float32)
     z = (np.arange(nz) * dx - cz).astype(np.float32)

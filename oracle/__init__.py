@@ -270,6 +270,46 @@ def track_trace(t, v):
     return a.value, d.value, e
 
 
+class MapTracker:
+    """The beat tracker of track_trace (activation at the first V >= -30 mV,
+    APD90 at the first later V <= Vmax - 0.9 (Vmax - Vrest), Vrest the V of
+    the first tracked step) for many voxels at once, fed one step at a time:
+    the per-voxel maps of the tissue's trackers.  update(k, v) takes the
+    step index k and every voxel's V after that step; results(times) gives
+    the step indices (act_step, rep_step, -1 where none) and, with the step
+    start times, the maps in ms (act, apd90; -1 where none)."""
+
+    def __init__(self, n):
+        self.act = np.full(n, -1, np.int64)
+        self.rep = np.full(n, -1, np.int64)
+        self.vrest = np.zeros(n)
+        self.vmax = np.zeros(n)
+        self.started = False
+
+    def update(self, k, v):
+        v = np.asarray(v, np.float64)
+        if not self.started:
+            self.vrest[:] = v
+            self.started = True
+        was = self.act >= 0
+        new = ~was & (v >= -30.0)
+        self.act[new] = k
+        self.vmax[new] = v[new]
+        live = was & (self.rep < 0)
+        self.vmax[live] = np.maximum(self.vmax[live], v[live])
+        down = live & (v <= self.vmax - 0.9 * (self.vmax - self.vrest))
+        self.rep[down] = k
+
+    def results(self, times=None):
+        if times is None:
+            return self.act.copy(), self.rep.copy()
+        times = np.asarray(times, np.float64)
+        act = np.where(self.act >= 0, times[np.maximum(self.act, 0)], -1.0)
+        rep_t = times[np.maximum(self.rep, 0)]
+        apd = np.where(self.rep >= 0, rep_t - act, -1.0)
+        return act, apd
+
+
 def qt_from_phi(t, phi, t_q, t_qrs_end, t_end):
     """QT of a pseudo-ECG (oracle restatement of the strand measurement):
     T peak = largest |phi| in [t_qrs_end, t_end); T end = first sample from

@@ -79,3 +79,25 @@ def test_qt_missing_t_end():
     phi = np.where(t > 200, 0.05, 0.0)  # never returns
     assert L.ecg_qt(phi, 0.0, 0.2, 50.0, 120.0, 500.0) == -1.0
     assert O.qt_from_phi(t, phi, 50.0, 120.0, 500.0) == -1.0
+
+
+def test_map_tracker_is_the_beat_tracker():
+    """oracle.MapTracker (the tissue maps' rule, many voxels fed one step at a
+    time) gives the activation and APD90 of track_trace on every trace of a
+    set that covers the cases: normal beats of varied timing, an EAD bump,
+    no repolarisation within the window, no activation, noise."""
+    t = np.arange(0, 800, 0.02)
+    rng = np.random.default_rng(4)
+    traces = [_ap(t, float(a), float(d)) for a, d in zip(rng.uniform(5, 200, 12), rng.uniform(150, 450, 12))]
+    traces += [_ap(t, 60.0, 400.0, ead_at=300.0, ead_mv=8.0), _ap(t, 30.0, 2000.0), np.full_like(t, -85.0),
+               _ap(t, 100.0, 250.0) + rng.normal(0.0, 0.3, t.size)]
+    V = np.array(traces)
+    tr = O.MapTracker(V.shape[0])
+    for k in range(t.size):
+        tr.update(k, V[:, k])
+    act, apd = tr.results(t)
+    a_step, r_step = tr.results()
+    for i, v in enumerate(traces):
+        a, d, _ = O.track_trace(t, v)
+        assert (act[i], apd[i]) == (a, d), i
+    assert a_step[-2] < 0 and act[-2] == -1.0 and apd[-3] == -1.0 and r_step[-3] < 0
