@@ -12,11 +12,11 @@
 #include "strands.hpp"
 
 namespace lqt {
-int strands_occupancy(bool fp32, int block);
+int strands_occupancy(bool fp32, int block, bool split);
 }
 using namespace lqt;
 
-constexpr int NMODES = 3;
+constexpr int NMODES = 4;
 
 struct lqt_strands {
     int n = 0;
@@ -29,18 +29,22 @@ struct lqt_strands {
     //       class (strands of 1..8 wavefronts) as parallel graph branches;
     //   1 = one strand per workgroup, one kernel at the largest block;
     //   2 = strands packed on whole wavefronts into 512-lane workgroups
-    //       (strands of similar step counts together), one kernel.
+    //       (strands of similar step counts together), one kernel;
+    //   3 = as 0, but strands of up to 4 wavefronts run split, each cell on
+    //       two lanes of two wavefronts (one per current group), at twice
+    //       the block size.
     struct Layout {
         int block, grid, first, count;  // bundles [first, first + count)
+        bool split;
     };
     std::vector<Layout> layout[NMODES];
     int mode = 0;
     bool tuned = false;
-    double tune_ms[NMODES] = {0, 0, 0};
+    double tune_ms[NMODES] = {0, 0, 0, 0};
     hipStream_t side[8] = {};
     hipEvent_t fork = nullptr, join[8] = {};
-    hipGraph_t graph_m[NMODES] = {nullptr, nullptr, nullptr};
-    hipGraphExec_t exec_m[NMODES] = {nullptr, nullptr, nullptr};
+    hipGraph_t graph_m[NMODES] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t exec_m[NMODES] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<lqt_strand_desc> desc;
     std::vector<uint8_t> label;  // per cell enum CellType
     std::vector<StrandDev> hdev;
@@ -75,6 +79,10 @@ struct lqt_strands {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double last_ms = 0;
     bool ran = false;
+    // a run enqueued by lqt_strands_run_async whose results have not been
+    // copied to the host yet, and the stream it went to
+    bool pending = false;
+    hipStream_t pending_stream = nullptr;
     // host copies of the results of the last run
     std::vector<double> h_act, h_apd, h_phit0, h_vrect0;
     std::vector<int64_t> h_nvrec;
@@ -122,14 +130,14 @@ static lqt_status launch_one(lqt_strands* s, int mode, int li, hipStream_t st) {
         a.nbundles = L.count;
         a.bundles = s->bund + L.first;
         a.queue = s->queue + li;
-        launch_strands(true, L.block, L.grid, &a, st);
+        launch_strands(true, L.block, L.split, L.grid, &a, st);
     } else {
         StrandsArgs<double> a;
         fill_sargs(s, a);
         a.nbundles = L.count;
         a.bundles = s->bund + L.first;
         a.queue = s->queue + li;
-        launch_strands(false, L.block, L.grid, &a, st);
+        launch_strands(false, L.block, L.split, L.grid, &a, st);
     }
     LQT_HIP(hipGetLastError());
     return LQT_OK;
@@ -166,12 +174,19 @@ static lqt_status graph_for(lqt_strands* s, int mode) {
     return LQT_OK;
 }
 
-static lqt_status run_mode(lqt_strands* s, int mode, double* ms_out) {
+// One run of a layout enqueued on stream st between the timing events.
+static lqt_status enqueue_mode(lqt_strands* s, int mode, hipStream_t st) {
     lqt_status e = graph_for(s, mode);
     if (e) return e;
-    LQT_HIP(hipEventRecord(s->e0, s->stream));
-    LQT_HIP(hipGraphLaunch(s->exec_m[mode], s->stream));
-    LQT_HIP(hipEventRecord(s->e1, s->stream));
+    LQT_HIP(hipEventRecord(s->e0, st));
+    LQT_HIP(hipGraphLaunch(s->exec_m[mode], st));
+    LQT_HIP(hipEventRecord(s->e1, st));
+    return LQT_OK;
+}
+
+static lqt_status run_mode(lqt_strands* s, int mode, double* ms_out) {
+    lqt_status e = enqueue_mode(s, mode, s->stream);
+    if (e) return e;
     LQT_HIP(hipStreamSynchronize(s->stream));
     float ms = 0;
     LQT_HIP(hipEventElapsedTime(&ms, s->e0, s->e1));
@@ -498,24 +513,28 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
         LQT_HIP(hipGetDevice(&dev));
         hipDeviceProp_t prop;
         LQT_HIP(hipGetDeviceProperties(&prop, dev));
-        auto grid_of = [&](int block, int count) {
-            const int occ = std::max(1, strands_occupancy(s->fp32, block));
+        auto grid_of = [&](int block, bool split, int count) {
+            const int occ = std::max(1, strands_occupancy(s->fp32, block, split));
             return std::max(1, std::min(count, occ * prop.multiProcessorCount));
         };
-        for (int i = 0; i < nstrands;) {
-            int j = i;
-            while (j < nstrands && waves(o0[j]) == waves(o0[i])) ++j;
-            const int b = 64 * waves(o0[i]);
-            s->layout[0].push_back({b, grid_of(b, j - i), (int)s->hbund.size(), j - i});
-            for (int k = i; k < j; ++k) {
-                Bundle bd = empty_bundle();
-                bundle_add(bd, o0[k], 0, waves(o0[k]));
-                s->hbund.push_back(bd);
+        // layouts 0 and 3: one persistent kernel per wavefront-count class
+        for (int mode : {0, 3}) {
+            for (int i = 0; i < nstrands;) {
+                int j = i;
+                while (j < nstrands && waves(o0[j]) == waves(o0[i])) ++j;
+                const bool split = mode == 3 && waves(o0[i]) <= 4;
+                const int b = 64 * waves(o0[i]) * (split ? 2 : 1);
+                s->layout[mode].push_back({b, grid_of(b, split, j - i), (int)s->hbund.size(), j - i, split});
+                for (int k = i; k < j; ++k) {
+                    Bundle bd = empty_bundle();
+                    bundle_add(bd, o0[k], 0, waves(o0[k]));
+                    s->hbund.push_back(bd);
+                }
+                i = j;
             }
-            i = j;
         }
         const int bmax = std::min(512, (maxl + 63) / 64 * 64);
-        s->layout[1].push_back({bmax, grid_of(bmax, nstrands), (int)s->hbund.size(), nstrands});
+        s->layout[1].push_back({bmax, grid_of(bmax, false, nstrands), (int)s->hbund.size(), nstrands, false});
         for (int k = 0; k < nstrands; ++k) {
             Bundle bd = empty_bundle();
             bundle_add(bd, o1[k], 0, waves(o1[k]));
@@ -523,7 +542,8 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
         }
         {
             std::vector<Bundle> pk = pack_bundles(H);
-            s->layout[2].push_back({512, grid_of(512, (int)pk.size()), (int)s->hbund.size(), (int)pk.size()});
+            s->layout[2].push_back({512, grid_of(512, false, (int)pk.size()), (int)s->hbund.size(), (int)pk.size(),
+                                    false});
             s->hbund.insert(s->hbund.end(), pk.begin(), pk.end());
         }
         set_kmax(s->hbund, H);
@@ -578,6 +598,7 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
 
 lqt_status lqt_strands_destroy(lqt_strands* s) {
     if (!s) return LQT_OK;
+    if (s->pending && s->pending_stream) hipStreamSynchronize(s->pending_stream);
     if (s->stream) hipStreamSynchronize(s->stream);
     for (int m = 0; m < NMODES; ++m) {
         if (s->exec_m[m]) hipGraphExecDestroy(s->exec_m[m]);
@@ -617,13 +638,14 @@ lqt_status lqt_strands_destroy(lqt_strands* s) {
     return LQT_OK;
 }
 
-lqt_status lqt_strands_run(lqt_strands* s) {
-    LQT_CHECK(s, LQT_ERR_ARG, "lqt_strands_run: null handle");
-    double ms = 0;
-    lqt_status e = run_mode(s, s->mode, &ms);
-    if (e) return e;
+// Results of the enqueued run to the host (waits for its stream first).
+static lqt_status fetch(lqt_strands* s) {
+    if (!s->pending) return LQT_OK;
+    s->pending = false;
+    LQT_HIP(hipStreamSynchronize(s->pending_stream));
+    float ms = 0;
+    LQT_HIP(hipEventElapsedTime(&ms, s->e0, s->e1));
     s->last_ms = ms;
-    // results to the host
     s->h_act.resize(s->ncells);
     s->h_apd.resize(s->ncells);
     s->h_ead.resize(s->ncells);
@@ -649,8 +671,35 @@ lqt_status lqt_strands_run(lqt_strands* s) {
     return LQT_OK;
 }
 
+lqt_status lqt_strands_run_async(lqt_strands* s, void* stream) {
+    LQT_CHECK(s, LQT_ERR_ARG, "lqt_strands_run_async: null handle");
+    LQT_CHECK(!s->pending, LQT_ERR_STATE, "lqt_strands_run_async: the previous run's results were not read yet");
+    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    lqt_status e = enqueue_mode(s, s->mode, st);
+    if (e) return e;
+    s->pending = true;
+    s->pending_stream = st;
+    s->ran = false;
+    return LQT_OK;
+}
+
+lqt_status lqt_strands_run(lqt_strands* s) {
+    LQT_CHECK(s, LQT_ERR_ARG, "lqt_strands_run: null handle");
+    if (s->pending) {  // an earlier asynchronous run: finish it first
+        lqt_status e = fetch(s);
+        if (e) return e;
+    }
+    lqt_status e = lqt_strands_run_async(s, nullptr);
+    if (e) return e;
+    return fetch(s);
+}
+
 lqt_status lqt_strands_results(lqt_strands* s, lqt_strand_result* out) {
     LQT_CHECK(s && out, LQT_ERR_ARG, "lqt_strands_results: bad argument");
+    {
+        lqt_status e_ = fetch(s);
+        if (e_) return e_;
+    }
     LQT_CHECK(s->ran, LQT_ERR_STATE, "lqt_strands_results: call lqt_strands_run first");
     for (int i = 0; i < s->n; ++i) {
         const lqt_strand_desc& d = s->desc[i];
@@ -697,6 +746,10 @@ lqt_status lqt_strands_results(lqt_strands* s, lqt_strand_result* out) {
 
 lqt_status lqt_strands_cell_results(lqt_strands* s, int i, double* act, double* apd90, int* ead) {
     LQT_CHECK(s && i >= 0 && i < s->n, LQT_ERR_ARG, "lqt_strands_cell_results: bad argument");
+    {
+        lqt_status e_ = fetch(s);
+        if (e_) return e_;
+    }
     LQT_CHECK(s->ran, LQT_ERR_STATE, "lqt_strands_cell_results: call lqt_strands_run first");
     const lqt_strand_desc& d = s->desc[i];
     for (int j = 0; j < d.ncell; ++j) {
@@ -710,6 +763,10 @@ lqt_status lqt_strands_cell_results(lqt_strands* s, int i, double* act, double* 
 
 lqt_status lqt_strands_ecg(lqt_strands* s, int i, double* phi, double* t0, int64_t* n) {
     LQT_CHECK(s && i >= 0 && i < s->n && n, LQT_ERR_ARG, "lqt_strands_ecg: bad argument");
+    {
+        lqt_status e_ = fetch(s);
+        if (e_) return e_;
+    }
     LQT_CHECK(s->ran, LQT_ERR_STATE, "lqt_strands_ecg: call lqt_strands_run first");
     *n = s->h_nphi[i];
     if (t0) *t0 = s->h_phit0[i];
@@ -720,6 +777,10 @@ lqt_status lqt_strands_ecg(lqt_strands* s, int i, double* phi, double* t0, int64
 
 lqt_status lqt_strands_vtrace(lqt_strands* s, int i, double* v, double* t0, int64_t* nrows) {
     LQT_CHECK(s && i >= 0 && i < s->n && nrows, LQT_ERR_ARG, "lqt_strands_vtrace: bad argument");
+    {
+        lqt_status e_ = fetch(s);
+        if (e_) return e_;
+    }
     LQT_CHECK(s->ran, LQT_ERR_STATE, "lqt_strands_vtrace: call lqt_strands_run first");
     *nrows = s->h_nvrec[i];
     if (t0) *t0 = s->h_vrect0[i];
@@ -732,6 +793,10 @@ lqt_status lqt_strands_vtrace(lqt_strands* s, int i, double* v, double* t0, int6
 
 lqt_status lqt_strands_final_state(lqt_strands* s, int i, double* out) {
     LQT_CHECK(s && out && i >= 0 && i < s->n, LQT_ERR_ARG, "lqt_strands_final_state: bad argument");
+    {
+        lqt_status e_ = fetch(s);
+        if (e_) return e_;
+    }
     LQT_CHECK(s->ran, LQT_ERR_STATE, "lqt_strands_final_state: call lqt_strands_run first");
     const lqt_strand_desc& d = s->desc[i];
     const int64_t c0 = d.cell_offset, n = d.ncell;
@@ -757,8 +822,10 @@ lqt_status lqt_strands_final_state(lqt_strands* s, int i, double* out) {
     return LQT_OK;
 }
 
-lqt_status lqt_strands_last_ms(const lqt_strands* s, double* ms) {
+lqt_status lqt_strands_last_ms(lqt_strands* s, double* ms) {
     LQT_CHECK(s && ms, LQT_ERR_ARG, "lqt_strands_last_ms: bad argument");
+    lqt_status e = fetch(s);
+    if (e) return e;
     *ms = s->last_ms;
     return LQT_OK;
 }
@@ -780,6 +847,10 @@ lqt_status lqt_strands_block(const lqt_strands* s, int* block) {
 
 lqt_status lqt_strands_autotune(lqt_strands* s, int trial_steps, int* mode, double* ms) {
     LQT_CHECK(s && trial_steps > 0, LQT_ERR_ARG, "lqt_strands_autotune: bad argument");
+    {
+        lqt_status e_ = fetch(s);  // an asynchronous run still in flight finishes first
+        if (e_) return e_;
+    }
     // The batch with every stop time scaled by the same factor, so that the
     // longest strand runs trial_steps steps and the strands keep their
     // relative lengths (the order in which they finish decides the layout),
@@ -799,7 +870,7 @@ lqt_status lqt_strands_autotune(lqt_strands* s, int trial_steps, int* mode, doub
     set_kmax(cb, cut);
     LQT_HIP(hipMemcpy(s->ddesc, cut.data(), sizeof(StrandDev) * s->n, hipMemcpyHostToDevice));
     LQT_HIP(hipMemcpy(s->bund, cb.data(), sizeof(Bundle) * cb.size(), hipMemcpyHostToDevice));
-    double t[NMODES] = {0, 0, 0};
+    double t[NMODES] = {0, 0, 0, 0};
     lqt_status e = LQT_OK;
     for (int m = 0; m < NMODES && !e; ++m) {
         double w = 0;
@@ -823,7 +894,7 @@ lqt_status lqt_strands_autotune(lqt_strands* s, int trial_steps, int* mode, doub
 }
 
 lqt_status lqt_strands_set_mode(lqt_strands* s, int mode) {
-    LQT_CHECK(s && mode >= 0 && mode < NMODES, LQT_ERR_ARG, "lqt_strands_set_mode: mode must be 0, 1 or 2");
+    LQT_CHECK(s && mode >= 0 && mode < NMODES, LQT_ERR_ARG, "lqt_strands_set_mode: mode must be 0, 1, 2 or 3");
     s->mode = mode;
     return LQT_OK;
 }

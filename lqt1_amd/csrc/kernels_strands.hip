@@ -24,19 +24,34 @@ __device__ __forceinline__ R wave_sum(R x) {
 // FP32: two waves per SIMD (state, trackers and step constants fit in 256
 // registers).  FP64 keeps its registers (one wave per SIMD for blocks up to
 // 256 lanes) rather than spilling to scratch.
-template <class R, int B>
+//
+// SPLIT: every cell runs on two lanes of two wavefronts, which the
+// dispatcher places on different SIMDs: the first B/2 lanes advance the
+// gates of current group A (tp_group_a), the other B/2 the gates of group B
+// (tp_group_b), and the two swap their groups' currents through LDS (a
+// second barrier per step); both then finish the step with the same
+// arithmetic, so they hold the same V and concentrations bit for bit and
+// the result is that of the single-lane update.  A step's dependent chain
+// per wavefront is about 0.6 of the single-lane one.  The bundle describes
+// the first half; outputs (trackers, pseudo-ECG, V rows) come from it.
+template <class R, int B, bool SPLIT>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 2 : 1))) void strands_kernel(
     StrandsArgs<R> A) {
-    constexpr int NW = B / 64;
+    constexpr int NH = SPLIT ? B / 2 : B;  // lanes of one half
+    constexpr int NW = NH / 64;            // wavefronts of one half (the bundle's)
     // one V segment per strand: its cells plus a mirrored end on each side
-    __shared__ R vbuf[2][B + 2 * MAXS];
+    __shared__ R vbuf[2][NH + 2 * MAXS];
     __shared__ float part[2][NW];
+    __shared__ R xa[SPLIT ? 7 : 1][SPLIT ? NH : 1];   // group A currents of each cell
+    __shared__ R xb[SPLIT ? 10 : 1][SPLIT ? NH : 1];  // group B currents of each cell
     __shared__ int s_next;
-    const int tid = threadIdx.x;
+    const int half = SPLIT ? (int)(threadIdx.x >= (unsigned)NH) : 0;
+    const int tid = (int)threadIdx.x - half * NH;  // lane within the half
     const int wid = tid >> 6;
+    const bool out_half = half == 0;               // the half that writes outputs
 
     for (;;) {
-        if (tid == 0) s_next = atomicAdd(A.queue, 1);
+        if (threadIdx.x == 0) s_next = atomicAdd(A.queue, 1);
         __syncthreads();
         const int q = s_next;
         __syncthreads();
@@ -51,7 +66,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
         const int n = has ? D.ncell : 0;
         const int j = tid - 64 * w0;        // cell of this lane in its strand
         const bool live = has && j < n;
-        const bool lead = has && j == 0;    // writes the strand's own outputs
+        const bool lead = out_half && has && j == 0;  // writes the strand's own outputs
         const int seg = 64 * w0 + 2 * slot; // LDS segment of the strand
         const int p = seg + 1 + j;          // this cell's LDS slot
         const int nws = (n + 63) >> 6;      // wavefronts of the strand
@@ -117,7 +132,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
                 s1_off = s1i < D.s1_count ? A.s1r[2 * (D.s1_first + s1i) + 1] : INT64_MAX;
             }
             // V^n into LDS with mirrored ends
-            if (live) {
+            if (live && out_half) {
                 vbuf[b][p] = v;
                 if (j == 0) vbuf[b][seg] = v;
                 if (j == n - 1) vbuf[b][seg + n + 1] = v;
@@ -140,19 +155,51 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
                 fin_idx = pend_idx;
                 pend = 0;
             }
-            if (run) {
-                if (live) {
-                    const R vl = vbuf[b][p - 1], vr = vbuf[b][p + 1], vc = vbuf[b][p];
-                    const R first = (j == 0 || j == n - 1) ? (vr - vl) * inv_dx : (vr - vl) * inv_2dx;
-                    const R second = (vr + vl - R(2) * vc) * inv_dx2;
-                    const R dvgap = RdD * first + RD * second;
-                    R ist = R(0);
-                    if (k >= s1_on) {  // in-cycle time in the S1 window
-                        if (j < D.nstim) ist = (R)D.stim_strength;
-                    } else if (k >= D.k_s2a && k < D.k_s2b && s2_cell) {
-                        ist = (R)D.s2_strength;
+            const bool act = run && live;
+            Pre<R> P;
+            GroupA<R> GA;
+            GroupB<R> GB;
+            R ist = R(0), dvgap = R(0);
+            if (act) {
+                const R vl = vbuf[b][p - 1], vr = vbuf[b][p + 1], vc = vbuf[b][p];
+                const R first = (j == 0 || j == n - 1) ? (vr - vl) * inv_dx : (vr - vl) * inv_2dx;
+                const R second = (vr + vl - R(2) * vc) * inv_dx2;
+                dvgap = RdD * first + RD * second;
+                if (k >= s1_on) {  // in-cycle time in the S1 window
+                    if (j < D.nstim) ist = (R)D.stim_strength;
+                } else if (k >= D.k_s2a && k < D.k_s2b && s2_cell) {
+                    ist = (R)D.s2_strength;
+                }
+                P = tp_pre(s, v, A.dt);
+                if (!SPLIT || half == 0) GA = tp_group_a(s, v, c, cc, A.dt, P);
+                if (!SPLIT || half == 1) GB = tp_group_b(s, v, c, cc, A.dt, P);
+                if (SPLIT) {
+                    if (half == 0) {
+                        xa[0][tid] = GA.INa; xa[1][tid] = GA.INaL; xa[2][tid] = GA.Ito; xa[3][tid] = GA.IKs;
+                        xa[4][tid] = GA.INaK; xa[5][tid] = GA.INab; xa[6][tid] = GA.IKb;
+                    } else {
+                        xb[0][tid] = GB.ICaL; xb[1][tid] = GB.ICaNa; xb[2][tid] = GB.ICaK; xb[3][tid] = GB.IKr;
+                        xb[4][tid] = GB.IK1; xb[5][tid] = GB.INaCa_i; xb[6][tid] = GB.INaCa_ss; xb[7][tid] = GB.IpCa;
+                        xb[8][tid] = GB.ICab; xb[9][tid] = GB.Jrel;
                     }
-                    v = tpord_update<R, false>(s, v, c, cc, A.dt, ist, dvgap, nullptr);
+                }
+            }
+            if (SPLIT) __syncthreads();
+            if (act) {
+                if (SPLIT) {
+                    if (half == 0) {
+                        GB.ICaL = xb[0][tid]; GB.ICaNa = xb[1][tid]; GB.ICaK = xb[2][tid]; GB.IKr = xb[3][tid];
+                        GB.IK1 = xb[4][tid]; GB.INaCa_i = xb[5][tid]; GB.INaCa_ss = xb[6][tid]; GB.IpCa = xb[7][tid];
+                        GB.ICab = xb[8][tid]; GB.Jrel = xb[9][tid];
+                    } else {
+                        GA.INa = xa[0][tid]; GA.INaL = xa[1][tid]; GA.Ito = xa[2][tid]; GA.IKs = xa[3][tid];
+                        GA.INaK = xa[4][tid]; GA.INab = xa[5][tid]; GA.IKb = xa[6][tid];
+                    }
+                }
+                v = tp_finish<R, false>(s, v, c, cc, A.dt, ist, dvgap, P, GA, GB, nullptr);
+            }
+            if (run && out_half) {
+                if (live) {
                     if (k >= D.k_wmax && (double)v >= wmax) wmax = (double)v;
 
                     // beat tracker, in the measured beat only (floor(t/BCL) == k)
@@ -204,16 +251,19 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
         }
 
         if (live) {
-            A.act[cell] = (tr.flags & 2) ? tr.act : -1.0;
-            A.apd[cell] = (tr.flags & 4) ? tr.rep - tr.act : -1.0;
-            int e = 0;
-            if (tr.flags & 8) e |= 1;
-            if ((tr.flags & 2) && !(tr.flags & 4)) e |= 2;
-            A.ead[cell] = e;
-            store_state(s, A.fin, cell);
-            A.finv[cell] = v;
-            if (j == 0) A.vmax[2 * (int64_t)sidx] = wmax;
-            if (j == n - 1) A.vmax[2 * (int64_t)sidx + 1] = wmax;
+            if (out_half) {
+                A.act[cell] = (tr.flags & 2) ? tr.act : -1.0;
+                A.apd[cell] = (tr.flags & 4) ? tr.rep - tr.act : -1.0;
+                int e = 0;
+                if (tr.flags & 8) e |= 1;
+                if ((tr.flags & 2) && !(tr.flags & 4)) e |= 2;
+                A.ead[cell] = e;
+                A.finv[cell] = v;
+                if (j == 0) A.vmax[2 * (int64_t)sidx] = wmax;
+                if (j == n - 1) A.vmax[2 * (int64_t)sidx + 1] = wmax;
+            }
+            if (SPLIT) store_state_half(s, A.fin, cell, half);
+            else store_state(s, A.fin, cell);
         }
         if (lead) {
             A.nvrec[sidx] = nvrow;
@@ -230,22 +280,33 @@ __global__ void zero_int_kernel(int* p, int n) {
 }
 
 template <class R>
-static void launch_b(int block, int grid, const StrandsArgs<R>& a, hipStream_t st) {
+static void launch_b(int block, bool split, int grid, const StrandsArgs<R>& a, hipStream_t st) {
+    if (split) {
+        switch (block) {
+            case 128: strands_kernel<R, 128, true><<<grid, 128, 0, st>>>(a); break;
+            case 256: strands_kernel<R, 256, true><<<grid, 256, 0, st>>>(a); break;
+            case 384: strands_kernel<R, 384, true><<<grid, 384, 0, st>>>(a); break;
+            default: strands_kernel<R, 512, true><<<grid, 512, 0, st>>>(a); break;
+        }
+        return;
+    }
     switch (block) {
-        case 64: strands_kernel<R, 64><<<grid, 64, 0, st>>>(a); break;
-        case 128: strands_kernel<R, 128><<<grid, 128, 0, st>>>(a); break;
-        case 192: strands_kernel<R, 192><<<grid, 192, 0, st>>>(a); break;
-        case 256: strands_kernel<R, 256><<<grid, 256, 0, st>>>(a); break;
-        case 320: strands_kernel<R, 320><<<grid, 320, 0, st>>>(a); break;
-        case 384: strands_kernel<R, 384><<<grid, 384, 0, st>>>(a); break;
-        case 448: strands_kernel<R, 448><<<grid, 448, 0, st>>>(a); break;
-        default: strands_kernel<R, 512><<<grid, 512, 0, st>>>(a); break;
+        case 64: strands_kernel<R, 64, false><<<grid, 64, 0, st>>>(a); break;
+        case 128: strands_kernel<R, 128, false><<<grid, 128, 0, st>>>(a); break;
+        case 192: strands_kernel<R, 192, false><<<grid, 192, 0, st>>>(a); break;
+        case 256: strands_kernel<R, 256, false><<<grid, 256, 0, st>>>(a); break;
+        case 320: strands_kernel<R, 320, false><<<grid, 320, 0, st>>>(a); break;
+        case 384: strands_kernel<R, 384, false><<<grid, 384, 0, st>>>(a); break;
+        case 448: strands_kernel<R, 448, false><<<grid, 448, 0, st>>>(a); break;
+        default: strands_kernel<R, 512, false><<<grid, 512, 0, st>>>(a); break;
     }
 }
 
-void launch_strands(bool fp32, int block, int grid, const void* args, hipStream_t st) {
-    if (fp32) launch_b<float>(block, grid, *(const StrandsArgs<float>*)args, st);
-    else launch_b<double>(block, grid, *(const StrandsArgs<double>*)args, st);
+// block: lanes of the workgroup (for a split launch both halves, 2 x 64 x
+// the bundle's wavefronts, 128..512).
+void launch_strands(bool fp32, int block, bool split, int grid, const void* args, hipStream_t st) {
+    if (fp32) launch_b<float>(block, split, grid, *(const StrandsArgs<float>*)args, st);
+    else launch_b<double>(block, split, grid, *(const StrandsArgs<double>*)args, st);
 }
 
 void launch_zero_int(int* p, int n, hipStream_t st) {
@@ -253,17 +314,23 @@ void launch_zero_int(int* p, int n, hipStream_t st) {
 }
 
 // Resident workgroups per CU of the strand kernel for a block size.
-int strands_occupancy(bool fp32, int block) {
+int strands_occupancy(bool fp32, int block, bool split) {
     int nb = 0;
     hipError_t e = hipSuccess;
-#define OCC(R, BB)                                                                                   \
-    case BB: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, strands_kernel<R, BB>, BB, 0); break;
-    if (fp32) {
-        switch (block) { OCC(float, 64) OCC(float, 128) OCC(float, 192) OCC(float, 256) OCC(float, 320)
-                         OCC(float, 384) OCC(float, 448) default: OCC(float, 512) }
+#define OCC(R, BB, SP)                                                                                     \
+    case BB: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, strands_kernel<R, BB, SP>, BB, 0); break;
+    if (split) {
+        if (fp32) {
+            switch (block) { OCC(float, 128, true) OCC(float, 256, true) OCC(float, 384, true) default: OCC(float, 512, true) }
+        } else {
+            switch (block) { OCC(double, 128, true) OCC(double, 256, true) OCC(double, 384, true) default: OCC(double, 512, true) }
+        }
+    } else if (fp32) {
+        switch (block) { OCC(float, 64, false) OCC(float, 128, false) OCC(float, 192, false) OCC(float, 256, false)
+                         OCC(float, 320, false) OCC(float, 384, false) OCC(float, 448, false) default: OCC(float, 512, false) }
     } else {
-        switch (block) { OCC(double, 64) OCC(double, 128) OCC(double, 192) OCC(double, 256) OCC(double, 320)
-                         OCC(double, 384) OCC(double, 448) default: OCC(double, 512) }
+        switch (block) { OCC(double, 64, false) OCC(double, 128, false) OCC(double, 192, false) OCC(double, 256, false)
+                         OCC(double, 320, false) OCC(double, 384, false) OCC(double, 448, false) default: OCC(double, 512, false) }
     }
 #undef OCC
     return e == hipSuccess ? nb : 1;

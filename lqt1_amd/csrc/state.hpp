@@ -85,6 +85,31 @@ __device__ __forceinline__ void store_state(const CellState<R>& s, const SoA<R>&
 #undef LQT_ST
 }
 
+// The gates that tp_group_b advances; a cell split over two lanes keeps the
+// current ones of these on its group-B lane and all other slots on its
+// group-A lane.
+__host__ __device__ constexpr bool b_gate(int slot) {
+    return slot == RS_d || slot == RS_ff || slot == RS_fs || slot == RS_fcaf || slot == RS_fcas || slot == RS_jca ||
+           slot == RS_nca || slot == RS_ffp || slot == RS_fcafp || slot == RS_xrf || slot == RS_xrs ||
+           slot == RS_xk1 || slot == RS_Jrelnp || slot == RS_Jrelp;
+}
+
+// store_state for one lane of a split cell: half 0 (group A) writes every
+// slot but the group-B gates, half 1 writes only those.
+template <class R>
+__device__ __forceinline__ void store_state_half(const CellState<R>& s, const SoA<R>& a, uint32_t k, int half) {
+    const uint32_t orr = k * (uint32_t)sizeof(R), od = k * 8u;
+#define LQT_ST(n) \
+    if (b_gate(RS_##n) == (half != 0)) st_nt(s.n, a.r + (size_t)RS_##n * a.pitch, orr);
+    LQT_RFIELDS(LQT_ST)
+#undef LQT_ST
+    if (half == 0) {
+#define LQT_ST(n) st_nt(s.n, a.d + (size_t)DS_##n * a.pitch, od);
+        LQT_DFIELDS(LQT_ST)
+#undef LQT_ST
+    }
+}
+
 // Chunk-major storage (the tissue's state and static voxel data): voxel k
 // belongs to chunk k / 64, the voxels of one wavefront, and a chunk keeps
 // its slots one after another, 64 values each.  A wavefront then reads and

@@ -78,7 +78,7 @@ struct StrandsArgs {
     double* vmax;              // per strand [2]: peak V of cell 0 / cell n-1 once time - s2_base >= s2_start
 };
 
-void launch_strands(bool fp32, int block, int grid, const void* args, hipStream_t st);
+void launch_strands(bool fp32, int block, bool split, int grid, const void* args, hipStream_t st);
 void launch_zero_int(int* p, int n, hipStream_t st);
 
 }  // namespace lqt

@@ -201,14 +201,37 @@ __device__ __forceinline__ R ncx(R na, R ca, R hna, R inv_hna, R inv_hca) {
     return JncxNa + R(2) * JncxCa;
 }
 
-// One dt of TPORd::update() for one cell.  v is V^n; returns V^{n+1}.
-template <class R, bool WANT_DIAG>
-__device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const Consts<R>& c, R dt,
-                                          R Ist, R dVgap, Diag* diag) {
-    using M = Math<R>;
-    const bool epi = (cls == CLS_EPI);
-    const bool mid = (cls == CLS_M);
+// The update of one cell in four parts.  tp_pre forms what both current
+// groups read from the step's start state; tp_group_a and tp_group_b advance
+// two disjoint sets of gates and return their currents; tp_finish sums the
+// currents in the reference's order and advances V, CaMK and the
+// concentrations.  Run one after the other (tpord_update) they are the
+// single-lane update; run on two lanes of different wavefronts that swap
+// their groups' currents, the two lanes hold the same V and concentrations
+// bit for bit, every expression being the same.
+template <class R>
+struct Pre {
+    R nai, ki, cansr, cajsr, CaMKt, nass, kss, cai, cass;
+    R ENa, EK, EKs, CaMKb, fCaMKp, vfrt, ndt, e1, e2, r1, r2;
+};
 
+// Group A: INa, INaL, Ito, IKs, INaK and the K/Na background currents, with
+// the gates sm sh sj mL hL hLp a iF iS ap iFp iSp xs1 xs2.
+template <class R>
+struct GroupA {
+    R INa, INaL, Ito, IKs, INaK, INab, IKb;
+};
+
+// Group B: ICaL/ICaNa/ICaK, IKr, IK1, INaCa, IpCa, ICab and Jrel, with the
+// gates d ff fs fcaf fcas jca nca ffp fcafp xrf xrs xk1 Jrelnp Jrelp.
+template <class R>
+struct GroupB {
+    R ICaL, ICaNa, ICaK, IKr, IK1, INaCa_i, INaCa_ss, IpCa, ICab, Jrel;
+};
+
+template <class R>
+__device__ __forceinline__ Pre<R> tp_pre(const CellState<R>& s, R v, R dt) {
+    using M = Math<R>;
     const R nai = R(s.nai), ki = R(s.ki), cansr = R(s.cansr), cajsr = R(s.cajsr), CaMKt = R(s.CaMKt);
     const R nass = s.nass, kss = s.kss, cai = s.cai, cass = s.cass;
 
@@ -223,6 +246,28 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
     const R fCaMKp = M::rcp(R(1) + R(k::KmCaMK) * M::rcp(CaMKa));  // fINaLp = fItop = fICaLp = fJrelp = fJupp
     const R vfrt = v * R(k::FRT);
     const R ndt = M::decay_scale(dt);
+    // the GHK exponentials of ICaL/ICaNa/ICaK shared with INab/ICab
+    const R e1 = M::ex(vfrt);
+    const R e2 = e1 * e1;
+    const R r1 = M::x_over_expm1(vfrt, e1);            // vfrt/(exp(vfrt)-1)
+    const R r2 = M::x_over_expm1(R(2) * vfrt, e2);     // 2vfrt/(exp(2vfrt)-1)
+    return Pre<R>{nai, ki, cansr, cajsr, CaMKt, nass, kss, cai, cass, ENa, EK, EKs, CaMKb, fCaMKp, vfrt, ndt,
+                  e1, e2, r1, r2};
+}
+
+template <class R>
+__device__ __forceinline__ GroupA<R> tp_group_a(CellState<R>& s, R v, int cls, const Consts<R>& c, R dt,
+                                                const Pre<R>& P) {
+    using M = Math<R>;
+    const bool epi = (cls == CLS_EPI);
+    const bool mid = (cls == CLS_M);
+    const R nai = P.nai, ki = P.ki, cansr = P.cansr, cajsr = P.cajsr, CaMKt = P.CaMKt;
+    const R nass = P.nass, kss = P.kss, cai = P.cai, cass = P.cass;
+    const R ENa = P.ENa, EK = P.EK, EKs = P.EKs, CaMKb = P.CaMKb, fCaMKp = P.fCaMKp;
+    const R vfrt = P.vfrt, ndt = P.ndt, e1 = P.e1, e2 = P.e2, r1 = P.r1, r2 = P.r2;
+    (void)epi; (void)mid; (void)nai; (void)ki; (void)cansr; (void)cajsr; (void)CaMKt; (void)nass; (void)kss;
+    (void)cai; (void)cass; (void)ENa; (void)EK; (void)EKs; (void)CaMKb; (void)fCaMKp; (void)vfrt; (void)ndt;
+    (void)e1; (void)e2; (void)r1; (void)r2;
 
     // ---- TP06 INa (:465-504)
     {
@@ -302,11 +347,75 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         Ito = Gto * (v - EK) * ((R(1) - fCaMKp) * s.a * i_ + fCaMKp * s.ap * ip);
     }
 
-    // ---- ICaL, ICaNa, ICaK (:573-652) and the GHK exponentials shared with INab/ICab
-    const R e1 = M::ex(vfrt);
-    const R e2 = e1 * e1;
-    const R r1 = M::x_over_expm1(vfrt, e1);            // vfrt/(exp(vfrt)-1)
-    const R r2 = M::x_over_expm1(R(2) * vfrt, e2);     // 2vfrt/(exp(2vfrt)-1)
+    // ---- IKs (:680-735)
+    R IKs;
+    {
+        const R xs1ss = M::rcp(R(1) + M::exl(v, -1.0 / 8.932, -11.60 / 8.932));
+        const R txs1 = R(817.3) + M::rcp(R(2.326e-4) * M::exl(v, 1.0 / 17.80, 48.28 / 17.80) +
+                                         R(0.001292) * M::exl(v, -1.0 / 230.0, -210.0 / 230.0));
+        s.xs1 = M::rl_slow(s.xs1, xs1ss, dt, M::rcp(txs1));
+        const R rate_xs2 = R(0.01) * M::exl(v, 1.0 / 20.0, -50.0 / 20.0) + R(0.0193) * M::exl(v, -1.0 / 31.0, -66.54 / 31.0);
+        s.xs2 = xs1ss - (xs1ss - s.xs2) * M::decay(ndt, rate_xs2);
+        const R KsCa = R(1) + R(0.6) * M::rcp(R(1) + M::pow14(R(3.8e-5) * M::rcp(cai)));
+        IKs = pick_class(c.gks, cls) * KsCa * s.xs1 * s.xs2 * (v - EKs);
+    }
+
+    // ---- INaK (:854-906)
+    R INaK;
+    {
+        const R Knai = R(9.073) * M::exl(v, -0.1550 * k::FRT / 3.0, 0.0);
+        const R Knao = R(27.78) * M::exl(v, (1.0 + 0.1550) * k::FRT / 3.0, 0.0);
+        const R P = R(k::eP) * M::rcp(R(1.0 + k::H / k::Khp) + nai * R(1.0 / k::Knap) + ki * R(1.0 / k::Kxkur));
+        const R ni = nai * M::rcp(Knai);
+        const R ni1 = R(1) + ni;
+        const R kk = R(1) + ki * R(1.0 / k::Kki);
+        const R deni = M::rcp(ni1 * ni1 * ni1 + kk * kk - R(1));
+        const R no = R(k::nao) * M::rcp(Knao);
+        const R no1 = R(1) + no;
+        const R deno = M::rcp(no1 * no1 * no1 + R(k::kok2) - R(1));
+        const R a1 = R(949.5) * ni * ni * ni * deni;
+        const R b1 = R(k::b1);
+        const R a2 = R(k::a2);
+        const R b2 = R(39.4) * no * no * no * deno;
+        const R a3 = R(k::a3num) * deno;
+        const R b3 = R(k::b3c) * P;
+        const R a4 = R(k::a4);
+        const R kr = ki * R(1.0 / k::Kki);
+        const R b4 = R(40.0) * kr * kr * deni;
+        const R x1 = a4 * a1 * a2 + b2 * b4 * b3 + a2 * b4 * b3 + b3 * a1 * a2;
+        const R x2 = b2 * b1 * b4 + a1 * a2 * a3 + a3 * b1 * b4 + a2 * a3 * b4;
+        const R x3 = a2 * a3 * a4 + b3 * b2 * b1 + b2 * b1 * a4 + a3 * a4 * b1;
+        const R x4 = b4 * b3 * b2 + a3 * a4 * a1 + b2 * a4 * a1 + b3 * b2 * a1;
+        const R inv_sum = M::rcp(x1 + x2 + x3 + x4);
+        const R JnakNa = R(3) * (x1 * a3 - x2 * b3) * inv_sum;
+        const R JnakK = R(2) * (x4 * b1 - x3 * a1) * inv_sum;
+        const R Pnak = epi ? R(30 * 0.9) : (mid ? R(30 * 0.7) : R(30));
+        INaK = Pnak * (JnakNa + JnakK);
+    }
+
+    // background (:908-923)
+    const R xkb = M::rcp(R(1) + M::exl(v, -1.0 / 18.34, 14.48 / 18.34));
+    const R GKb = epi ? R(0.003 * 0.6) : R(0.003);
+    const R IKb = GKb * xkb * (v - EK);
+    const R INab = R(3.75e-10 * k::F) * r1 * (nai * e1 - R(k::nao));
+    return GroupA<R>{INa, INaL, Ito, IKs, INaK, INab, IKb};
+}
+
+template <class R>
+__device__ __forceinline__ GroupB<R> tp_group_b(CellState<R>& s, R v, int cls, const Consts<R>& c, R dt,
+                                                const Pre<R>& P) {
+    using M = Math<R>;
+    const bool epi = (cls == CLS_EPI);
+    const bool mid = (cls == CLS_M);
+    const R nai = P.nai, ki = P.ki, cansr = P.cansr, cajsr = P.cajsr, CaMKt = P.CaMKt;
+    const R nass = P.nass, kss = P.kss, cai = P.cai, cass = P.cass;
+    const R ENa = P.ENa, EK = P.EK, EKs = P.EKs, CaMKb = P.CaMKb, fCaMKp = P.fCaMKp;
+    const R vfrt = P.vfrt, ndt = P.ndt, e1 = P.e1, e2 = P.e2, r1 = P.r1, r2 = P.r2;
+    (void)epi; (void)mid; (void)nai; (void)ki; (void)cansr; (void)cajsr; (void)CaMKt; (void)nass; (void)kss;
+    (void)cai; (void)cass; (void)ENa; (void)EK; (void)EKs; (void)CaMKb; (void)fCaMKp; (void)vfrt; (void)ndt;
+    (void)e1; (void)e2; (void)r1; (void)r2;
+
+    // ---- ICaL, ICaNa, ICaK (:573-652)
     R ICaL, ICaNa, ICaK;
     {
         const R vs = v + c.dss_shift;
@@ -373,19 +482,6 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         IKr = GKr * xr * rkr * (v - EK);  // sqrt(ko/5.4) == 1
     }
 
-    // ---- IKs (:680-735)
-    R IKs;
-    {
-        const R xs1ss = M::rcp(R(1) + M::exl(v, -1.0 / 8.932, -11.60 / 8.932));
-        const R txs1 = R(817.3) + M::rcp(R(2.326e-4) * M::exl(v, 1.0 / 17.80, 48.28 / 17.80) +
-                                         R(0.001292) * M::exl(v, -1.0 / 230.0, -210.0 / 230.0));
-        s.xs1 = M::rl_slow(s.xs1, xs1ss, dt, M::rcp(txs1));
-        const R rate_xs2 = R(0.01) * M::exl(v, 1.0 / 20.0, -50.0 / 20.0) + R(0.0193) * M::exl(v, -1.0 / 31.0, -66.54 / 31.0);
-        s.xs2 = xs1ss - (xs1ss - s.xs2) * M::decay(ndt, rate_xs2);
-        const R KsCa = R(1) + R(0.6) * M::rcp(R(1) + M::pow14(R(3.8e-5) * M::rcp(cai)));
-        IKs = pick_class(c.gks, cls) * KsCa * s.xs1 * s.xs2 * (v - EKs);
-    }
-
     // ---- IK1 (:738-751)
     R IK1;
     {
@@ -413,46 +509,44 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
         INaCa_ss = R(0.2) * Gncx * allo_ss * ncx<R>(nass, cass, hna, inv_hna, inv_hca);
     }
 
-    // ---- INaK (:854-906)
-    R INaK;
-    {
-        const R Knai = R(9.073) * M::exl(v, -0.1550 * k::FRT / 3.0, 0.0);
-        const R Knao = R(27.78) * M::exl(v, (1.0 + 0.1550) * k::FRT / 3.0, 0.0);
-        const R P = R(k::eP) * M::rcp(R(1.0 + k::H / k::Khp) + nai * R(1.0 / k::Knap) + ki * R(1.0 / k::Kxkur));
-        const R ni = nai * M::rcp(Knai);
-        const R ni1 = R(1) + ni;
-        const R kk = R(1) + ki * R(1.0 / k::Kki);
-        const R deni = M::rcp(ni1 * ni1 * ni1 + kk * kk - R(1));
-        const R no = R(k::nao) * M::rcp(Knao);
-        const R no1 = R(1) + no;
-        const R deno = M::rcp(no1 * no1 * no1 + R(k::kok2) - R(1));
-        const R a1 = R(949.5) * ni * ni * ni * deni;
-        const R b1 = R(k::b1);
-        const R a2 = R(k::a2);
-        const R b2 = R(39.4) * no * no * no * deno;
-        const R a3 = R(k::a3num) * deno;
-        const R b3 = R(k::b3c) * P;
-        const R a4 = R(k::a4);
-        const R kr = ki * R(1.0 / k::Kki);
-        const R b4 = R(40.0) * kr * kr * deni;
-        const R x1 = a4 * a1 * a2 + b2 * b4 * b3 + a2 * b4 * b3 + b3 * a1 * a2;
-        const R x2 = b2 * b1 * b4 + a1 * a2 * a3 + a3 * b1 * b4 + a2 * a3 * b4;
-        const R x3 = a2 * a3 * a4 + b3 * b2 * b1 + b2 * b1 * a4 + a3 * a4 * b1;
-        const R x4 = b4 * b3 * b2 + a3 * a4 * a1 + b2 * a4 * a1 + b3 * b2 * a1;
-        const R inv_sum = M::rcp(x1 + x2 + x3 + x4);
-        const R JnakNa = R(3) * (x1 * a3 - x2 * b3) * inv_sum;
-        const R JnakK = R(2) * (x4 * b1 - x3 * a1) * inv_sum;
-        const R Pnak = epi ? R(30 * 0.9) : (mid ? R(30 * 0.7) : R(30));
-        INaK = Pnak * (JnakNa + JnakK);
-    }
-
-    // ---- background currents (:908-923)
-    const R xkb = M::rcp(R(1) + M::exl(v, -1.0 / 18.34, 14.48 / 18.34));
-    const R GKb = epi ? R(0.003 * 0.6) : R(0.003);
-    const R IKb = GKb * xkb * (v - EK);
-    const R INab = R(3.75e-10 * k::F) * r1 * (nai * e1 - R(k::nao));
+    // background (:908-923)
     const R ICab = R(2.5e-8 * 2.0 * k::F) * r2 * (cai * e2 - R(0.341 * k::cao));
     const R IpCa = R(0.0005) * cai * M::rcp(R(0.0005) + cai);
+    // Jrel (:946-979), RL gates Jrelnp/Jrelp driven by ICaL
+    const R rj = R(1.5) * M::rcp(cajsr);
+    const R rj2 = rj * rj;
+    const R rj4 = rj2 * rj2;
+    const R jrel_den = M::rcp(R(1) + rj4 * rj4);
+    const R mfac = mid ? R(1.7) : R(1);
+    const R Jrel_inf = R(0.5 * 4.75) * (-ICaL) * jrel_den * mfac;
+    const R q_rel = M::rcp(R(1) + R(0.0123) * M::rcp(cajsr));
+    R tau_rel = R(4.75) * q_rel;
+    tau_rel = tau_rel < R(0.005) ? R(0.005) : tau_rel;
+    s.Jrelnp = Jrel_inf - (Jrel_inf - s.Jrelnp) * M::decay(ndt, M::rcp(tau_rel));
+    const R Jrel_infp = R(0.5 * 1.25 * 4.75) * (-ICaL) * jrel_den * mfac;
+    R tau_relp = R(1.25 * 4.75) * q_rel;
+    tau_relp = tau_relp < R(0.005) ? R(0.005) : tau_relp;
+    s.Jrelp = Jrel_infp - (Jrel_infp - s.Jrelp) * M::decay(ndt, M::rcp(tau_relp));
+    const R Jrel = (R(1) - fCaMKp) * s.Jrelnp + fCaMKp * s.Jrelp;
+    return GroupB<R>{ICaL, ICaNa, ICaK, IKr, IK1, INaCa_i, INaCa_ss, IpCa, ICab, Jrel};
+}
+
+template <class R, bool WANT_DIAG>
+__device__ __forceinline__ R tp_finish(CellState<R>& s, R v, int cls, const Consts<R>& c, R dt, R Ist, R dVgap,
+                                       const Pre<R>& P, const GroupA<R>& GA, const GroupB<R>& GB, Diag* diag) {
+    using M = Math<R>;
+    const bool epi = (cls == CLS_EPI);
+    const bool mid = (cls == CLS_M);
+    const R nai = P.nai, ki = P.ki, cansr = P.cansr, cajsr = P.cajsr, CaMKt = P.CaMKt;
+    const R nass = P.nass, kss = P.kss, cai = P.cai, cass = P.cass;
+    const R ENa = P.ENa, EK = P.EK, EKs = P.EKs, CaMKb = P.CaMKb, fCaMKp = P.fCaMKp;
+    const R vfrt = P.vfrt, ndt = P.ndt, e1 = P.e1, e2 = P.e2, r1 = P.r1, r2 = P.r2;
+    (void)epi; (void)mid; (void)nai; (void)ki; (void)cansr; (void)cajsr; (void)CaMKt; (void)nass; (void)kss;
+    (void)cai; (void)cass; (void)ENa; (void)EK; (void)EKs; (void)CaMKb; (void)fCaMKp; (void)vfrt; (void)ndt;
+    (void)e1; (void)e2; (void)r1; (void)r2;
+    const R INa = GA.INa, INaL = GA.INaL, Ito = GA.Ito, IKs = GA.IKs, INaK = GA.INaK, INab = GA.INab, IKb = GA.IKb;
+    const R ICaL = GB.ICaL, ICaNa = GB.ICaNa, ICaK = GB.ICaK, IKr = GB.IKr, IK1 = GB.IK1, INaCa_i = GB.INaCa_i,
+            INaCa_ss = GB.INaCa_ss, IpCa = GB.IpCa, ICab = GB.ICab, Jrel = GB.Jrel;
 
     // ---- voltage (:927-935)
     const R INaCa = INaCa_i + INaCa_ss;
@@ -478,21 +572,6 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
     const R JdiffK = (kss - ki) * R(0.5);
     const R Jdiff = (cass - cai) * R(5.0);
 
-    const R rj = R(1.5) * M::rcp(cajsr);
-    const R rj2 = rj * rj;
-    const R rj4 = rj2 * rj2;
-    const R jrel_den = M::rcp(R(1) + rj4 * rj4);
-    const R mfac = mid ? R(1.7) : R(1);
-    const R Jrel_inf = R(0.5 * 4.75) * (-ICaL) * jrel_den * mfac;
-    const R q_rel = M::rcp(R(1) + R(0.0123) * M::rcp(cajsr));
-    R tau_rel = R(4.75) * q_rel;
-    tau_rel = tau_rel < R(0.005) ? R(0.005) : tau_rel;
-    s.Jrelnp = Jrel_inf - (Jrel_inf - s.Jrelnp) * M::decay(ndt, M::rcp(tau_rel));
-    const R Jrel_infp = R(0.5 * 1.25 * 4.75) * (-ICaL) * jrel_den * mfac;
-    R tau_relp = R(1.25 * 4.75) * q_rel;
-    tau_relp = tau_relp < R(0.005) ? R(0.005) : tau_relp;
-    s.Jrelp = Jrel_infp - (Jrel_infp - s.Jrelp) * M::decay(ndt, M::rcp(tau_relp));
-    const R Jrel = (R(1) - fCaMKp) * s.Jrelnp + fCaMKp * s.Jrelp;
 
     const R upf = epi ? R(1.3) : R(1);
     const R Jupnp = upf * R(0.004375) * cai * M::rcp(cai + R(0.00092));
@@ -533,6 +612,16 @@ __device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const C
     s.cansr += double(dt * dcansr);
     s.cajsr += double(dt * dcajsr);
     return vnew;
+}
+
+// One dt of TPORd::update() for one cell.  v is V^n; returns V^{n+1}.
+template <class R, bool WANT_DIAG>
+__device__ __forceinline__ R tpord_update(CellState<R>& s, R v, int cls, const Consts<R>& c, R dt,
+                                          R Ist, R dVgap, Diag* diag) {
+    const Pre<R> P = tp_pre(s, v, dt);
+    const GroupA<R> GA = tp_group_a(s, v, cls, c, dt, P);
+    const GroupB<R> GB = tp_group_b(s, v, cls, c, dt, P);
+    return tp_finish<R, WANT_DIAG>(s, v, cls, c, dt, Ist, dVgap, P, GA, GB, diag);
 }
 
 }  // namespace lqt

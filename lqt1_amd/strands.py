@@ -94,11 +94,13 @@ class Strands:
 
     def autotune(self, trial_steps=2000):
         """Time every launch layout (0: per-size classes, 1: one strand per
-        workgroup at the largest block, 2: packed bundles) on the batch with
-        its stop times scaled so that the longest strand runs trial_steps
-        steps, and keep the fastest; returns (mode, [ms per layout])."""
+        workgroup at the largest block, 2: packed bundles, 3: per-size
+        classes with each cell of strands up to 256 cells split over two
+        lanes) on the batch with its stop times scaled so that the longest
+        strand runs trial_steps steps, and keep the fastest; returns (mode,
+        [ms per layout])."""
         mode = C.c_int()
-        ms = np.zeros(3)
+        ms = np.zeros(4)
         L.check(L.lib().lqt_strands_autotune(self.h, int(trial_steps), C.byref(mode), L.ptr(ms)))
         return mode.value, ms.tolist()
 

@@ -66,7 +66,7 @@ def test_strands_at_wavefront_edges_match_oracle(lqt, oracle):
     specs = [lqt.StrandSpec(lqt.transmural(n), lqt.preset("cpu_default"), bcl=400.0, nbeats=1, stim_start=5.0,
                             nstim=min(3, n)) for n in lengths]
     ref = [_oracle(O, s) for s in specs]
-    for mode in (0, 2):
+    for mode in (0, 2, 3):
         st = lqt.Strands(specs, lqt.FP64, 0.02)
         st.set_mode(mode)
         st.run()

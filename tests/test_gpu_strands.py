@@ -45,7 +45,7 @@ def oracle_runs(oracle, lqt):
     return [_oracle(oracle, s) for s in _specs(lqt)]
 
 
-@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("mode", [0, 2, 3])
 def test_strands_fp64_match_oracle(lqt, oracle_runs, mode):
     """Layout 2 puts the 57-, 100- and 80-cell strands (60001, 50001 and
     50001 steps) in one workgroup and the 130-cell strand in another."""
@@ -94,13 +94,14 @@ def test_strands_fp32_within_gates(lqt, oracle_runs):
 
 def test_launch_layouts_agree_and_autotune(lqt):
     """Every launch layout (per-size classes in parallel branches, one kernel
-    at the largest block, strands packed several to a workgroup) gives
-    identical per-strand results; each layout runs on its own handle so that
-    a strand a layout skipped would show up as missing output, and every
+    at the largest block, strands packed several to a workgroup, per-size
+    classes with each cell split over two lanes) gives identical per-strand
+    results and final states; each layout runs on its own handle so that a
+    strand a layout skipped would show up as missing output, and every
     strand reports its full step count."""
     specs = _specs(lqt)
     outs = []
-    for mode in (0, 1, 2):
+    for mode in (0, 1, 2, 3):
         h = lqt.Strands(specs, lqt.FP32, 0.02)
         h.set_mode(mode)
         h.run()
@@ -111,7 +112,8 @@ def test_launch_layouts_agree_and_autotune(lqt):
                 t += 0.02
                 n += 1
             assert r["steps"] == n, (mode, i, r["steps"], n)
-        outs.append([h.cell_results(i) for i in range(len(specs))] + [h.ecg(i)[1] for i in range(len(specs))])
+        outs.append([h.cell_results(i) for i in range(len(specs))] + [h.ecg(i)[1] for i in range(len(specs))] +
+                    [h.final_state(i) for i in range(len(specs))])
         h.close()
     st = lqt.Strands(specs, lqt.FP32, 0.02)
     for other in outs[1:]:
@@ -123,13 +125,13 @@ def test_launch_layouts_agree_and_autotune(lqt):
             else:
                 assert np.array_equal(a, b)
     mode, ms = st.autotune(500)
-    assert mode in (0, 1, 2) and ms[mode] == min(ms)
+    assert mode in (0, 1, 2, 3) and ms[mode] == min(ms)
     st.run()
     act, _, _ = st.cell_results(0)
     assert np.array_equal(act, outs[0][0][0])
 
 
-@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("mode", [0, 2, 3])
 def test_graph_replays_are_identical(lqt, mode):
     specs = _specs(lqt)[1:3]
     st = lqt.Strands(specs, lqt.FP32, 0.02)
