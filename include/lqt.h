@@ -89,6 +89,12 @@ lqt_status lqt_cells_destroy(lqt_cells *c);
 /* nsteps calls of TPORd::update() per cell with fixed Istim / dVgap_dt
  * (host arrays of n, or NULL for 0).  Replaces setIstim+setDVgap_dt+update. */
 lqt_status lqt_cells_step(lqt_cells *c, int nsteps, const double *istim, const double *dvgap);
+/* Same, enqueued on the caller's HIP stream (hipStream_t as void*, NULL for
+ * the handle's own stream) without waiting: istim and dvgap are DEVICE
+ * arrays of n doubles (or NULL), read while the kernel runs.  The handle's
+ * other calls use its own stream: the caller synchronises `stream` before
+ * reading states back or stepping again from another stream. */
+lqt_status lqt_cells_step_async(lqt_cells *c, int nsteps, const double *istim, const double *dvgap, void *stream);
 /* All 40 states per cell in the order of TPORd::outputAllStates
  * (TPORd.cc:200-252), out[n*40]. */
 lqt_status lqt_cells_get_state(lqt_cells *c, double *out);
@@ -154,6 +160,18 @@ lqt_status lqt_tissue_create_from_file(const char *path, int nx, int ny, int nz,
                                        double d_fibre, double d_cross, const lqt_params *params,
                                        lqt_precision prec, double dt, lqt_tissue **out);
 lqt_status lqt_tissue_info(const lqt_tissue *t, int64_t *nvox, int64_t *nghost);
+/* Host only: the voxel list reader of lqt_tissue_create_from_file
+ * (GPU3D.cu:1276-1297) into caller grids label[nx*ny*nz] (LQT_NONE where no
+ * voxel is listed) and fibre[nx*ny*nz*3] (may be NULL); *nvox (may be NULL)
+ * receives the tissue voxel count.  A voxel outside the grid is refused. */
+lqt_status lqt_geometry_read_file(const char *path, int nx, int ny, int nz, uint8_t *label, float *fibre,
+                                  int64_t *nvox);
+/* Host only: the stimulation profile reader of lqt_tissue_set_stimulus_file
+ * (GPU3D.cu:1691-1721): the grid indices of the sites inside the grid, in
+ * file order, and their offsets 1000 * t_act in ms; *n receives the count
+ * (call with cells and offsets_ms NULL to query; cap is their length). */
+lqt_status lqt_stimulus_read_file(const char *path, int nx, int ny, int nz, int64_t cap, int64_t *cells,
+                                  float *offsets_ms, int64_t *n);
 /* Host only (no device needed): compaction and ghost mirror map of a label
  * grid as built by lqt_tissue_create (GPU3D.cu:1301-1333, 1487-1604).
  * g[nx*ny*nz] receives k+1 for tissue voxel k (x,y,z order), -(k+1) for a
@@ -167,9 +185,13 @@ lqt_status lqt_tissue_set_stimulus(lqt_tissue *t, const lqt_stim_protocol *p, in
                                    int64_t ns2, const int64_t *s2_cells);
 /* Reads stimulationprofile.cnd ("x y z f f f t_act[s] s", GPU3D.cu:1691-1721). */
 lqt_status lqt_tissue_set_stimulus_file(lqt_tissue *t, const lqt_stim_protocol *p, const char *path);
-/* Advance nsteps steps of dt.  Returns LQT_ERR_RANGE if V became non-finite. */
+/* Advance nsteps steps of dt and wait for them.  Returns LQT_ERR_RANGE if V
+ * became non-finite. */
 lqt_status lqt_tissue_step(lqt_tissue *t, int64_t nsteps);
-/* Same, on the caller's HIP stream (hipStream_t passed as void*). */
+/* Same, enqueued on the caller's HIP stream (hipStream_t passed as void*,
+ * NULL for the tissue's own stream); returns without waiting for the device
+ * and records no timing events.  The non-finite check of lqt_tissue_step is
+ * then the caller's, through lqt_tissue_v_guard. */
 lqt_status lqt_tissue_step_async(lqt_tissue *t, int64_t nsteps, void *stream);
 lqt_status lqt_tissue_time(const lqt_tissue *t, double *time_ms, int64_t *step);
 /* V of every tissue voxel in compacted (x,y,z) order, out[nvox]. */
@@ -217,10 +239,11 @@ lqt_status lqt_tissue_get_variant(const lqt_tissue *t, int *i);
  * *best; us[count] (may be NULL) receives each shape's mean time per step
  * over its two trials. */
 lqt_status lqt_tissue_autotune(lqt_tissue *t, int trial_steps, int *best, double *us);
-/* Average device time of the fused step kernel over the last call of
- * lqt_tissue_step, in microseconds (HIP events on the tissue's stream, on
- * every 16th step).  The step kernel refreshes the ghost cells itself, so
- * *us_ghost_kernel is 0. */
+/* Mean device time per step of the last call of lqt_tissue_step, in
+ * microseconds: one pair of HIP events on the tissue's stream brackets all
+ * of that call's launches (one fused step kernel per step, plus the tracker
+ * kernel on tracked steps), divided by the step count.  The step kernel
+ * refreshes the ghost cells itself, so *us_ghost_kernel is 0. */
 lqt_status lqt_tissue_last_kernel_us(const lqt_tissue *t, double *us_step_kernel, double *us_ghost_kernel);
 /* Algorithmic HBM bytes of one fused step kernel launch (per voxel state,
  * V and static data, read and written once; per ghost cell its index read
@@ -279,6 +302,12 @@ lqt_status lqt_strands_destroy(lqt_strands *s);
 /* Runs every strand to its stop time in one persistent launch (captured in
  * a HIP graph on first use).  Resets the strands to the initial state first. */
 lqt_status lqt_strands_run(lqt_strands *s);
+/* Same run enqueued on the caller's HIP stream (hipStream_t as void*, NULL
+ * for the handle's own stream) without waiting.  The first call of the
+ * result functions below (or lqt_strands_last_ms, lqt_strands_autotune,
+ * lqt_strands_run) waits for that stream and copies the results; a second
+ * lqt_strands_run_async before then is refused (LQT_ERR_STATE). */
+lqt_status lqt_strands_run_async(lqt_strands *s, void *stream);
 lqt_status lqt_strands_results(lqt_strands *s, lqt_strand_result *out);
 /* Per-cell activation time, APD90 and EAD flag of strand i (ncell each). */
 lqt_status lqt_strands_cell_results(lqt_strands *s, int i, double *act, double *apd90, int *ead);
@@ -291,17 +320,21 @@ lqt_status lqt_strands_vtrace(lqt_strands *s, int i, double *v, double *t0, int6
 /* Final 40-state vectors of strand i (ncell*40). */
 lqt_status lqt_strands_final_state(lqt_strands *s, int i, double *out);
 /* Device time of the last lqt_strands_run (ms, HIP events). */
-lqt_status lqt_strands_last_ms(const lqt_strands *s, double *ms);
+lqt_status lqt_strands_last_ms(lqt_strands *s, double *ms);
 /* Largest work-group size of the active launch layout (64..512). */
 lqt_status lqt_strands_block(const lqt_strands *s, int *block);
 /* Launch layouts: mode 0 runs one strand per workgroup, one persistent
  * kernel per block-size class (strands of 1..8 wavefronts, each class at its
  * own block size, all classes as parallel branches of one graph); mode 1 runs
  * one strand per workgroup at the largest block; mode 2 packs strands of
- * similar step counts on whole wavefronts into 512-lane workgroups.  The
- * autotuner times every layout on the batch with all stop times scaled so
- * that the longest strand runs trial_steps steps, and keeps the fastest
- * (ms[3] receives the trial times). */
+ * similar step counts on whole wavefronts into 512-lane workgroups; mode 3 is
+ * mode 0 with every strand of up to 4 wavefronts (256 cells) split: each cell
+ * on two lanes of two wavefronts, one per half of the ionic currents, which
+ * exchange their currents every step (same results bit for bit, a shorter
+ * step when the strands leave SIMDs idle).  The autotuner times every layout
+ * on the batch with all stop times scaled so that the longest strand runs
+ * trial_steps steps, and keeps the fastest (ms[4] receives the trial
+ * times). */
 lqt_status lqt_strands_autotune(lqt_strands *s, int trial_steps, int *mode, double *ms);
 lqt_status lqt_strands_set_mode(lqt_strands *s, int mode);
 
@@ -381,8 +414,9 @@ lqt_status lqt_mlp_train(lqt_mlp *m, int64_t n, const float *x, const float *y, 
  * band_ms of qt_threshold or whose EAD probability lies in (p_lo, p_hi) is
  * re-run as an OneD.cc strand (S1 -52 pA/pF, 1 ms, cells 0-2, from 50 ms;
  * dx 0.15 mm, D 0.154; ENDO (1 - mfrac)/2, then MCELL mfrac, then EPI;
- * SUPREP at dose * dose_max_fmol) by the batched solver, whose QT and EAD
- * replace the prediction. */
+ * SUPREP at dose * dose_max_fmol; PCa x pca_iso under ISO, 0 meaning the
+ * TPORd.cc value 1.6) by the batched solver, whose QT and EAD replace the
+ * prediction. */
 typedef struct {
     double bcl;            /* ms                                          */
     int ncell;             /* strand length, 2..512                       */
@@ -394,6 +428,7 @@ typedef struct {
     double dose_max_fmol;  /* dose 1 = this many fmol                     */
     int chunk;             /* strands per solver launch; 0 = 4096         */
     int autotune_steps;    /* launch-layout trial steps per chunk; 0 = 2000 */
+    double pca_iso;        /* PCa multiplier under ISO (TPORd.cc:629 1.6; GPU3D.cu:767 4) */
 } lqt_sweep_desc;
 
 typedef struct {
@@ -405,7 +440,10 @@ typedef struct {
 
 /* Grid point (a, b, c) = (iks[a], dose[b], iso[c]) at index (a*n_dose + b)*n_iso + c.
  * Per point: qt (ms), ead and long_qt decisions, rechecked flag, and
- * qt_err = predicted - solver QT on re-checked points (NaN elsewhere). */
+ * qt_err = predicted - solver QT on re-checked points (NaN elsewhere).  A
+ * re-checked point whose strand has no T end within the beat (QT not
+ * measurable, lqt_strand_result.qt = -1) gets qt = NaN and long_qt = 1:
+ * long_qt = !(qt <= qt_threshold). */
 lqt_status lqt_surrogate_sweep(lqt_mlp *net, int n_iks, const double *iks, int n_dose, const double *dose,
                                int n_iso, const int *iso, const lqt_sweep_desc *desc, lqt_precision prec,
                                double *qt, uint8_t *ead, uint8_t *long_qt, uint8_t *rechecked, double *qt_err,

@@ -71,7 +71,8 @@ class StrandDesc(C.Structure):
 class SweepDesc(C.Structure):
     _fields_ = [("bcl", C.c_double), ("ncell", C.c_int), ("mfrac", C.c_double), ("nbeats", C.c_int),
                 ("qt_threshold", C.c_double), ("band_ms", C.c_double), ("p_lo", C.c_double), ("p_hi", C.c_double),
-                ("dose_max_fmol", C.c_double), ("chunk", C.c_int), ("autotune_steps", C.c_int)]
+                ("dose_max_fmol", C.c_double), ("chunk", C.c_int), ("autotune_steps", C.c_int),
+                ("pca_iso", C.c_double)]
 
 
 class SweepStats(C.Structure):
@@ -127,6 +128,7 @@ def lib():
     _sig(L, "lqt_cells_create", [C.c_int, _VP, P(Params), C.c_int, C.c_int, C.c_double, P(_VP)])
     _sig(L, "lqt_cells_destroy", [_VP])
     _sig(L, "lqt_cells_step", [_VP, C.c_int, _VP, _VP])
+    _sig(L, "lqt_cells_step_async", [_VP, C.c_int, _VP, _VP, _VP])
     _sig(L, "lqt_cells_get_state", [_VP, _VP])
     _sig(L, "lqt_cells_set_state", [_VP, _VP])
     _sig(L, "lqt_cells_get_v", [_VP, _VP])
@@ -160,6 +162,7 @@ def lib():
     _sig(L, "lqt_strands_create", [C.c_int, _VP, _VP, _VP, P(Params), C.c_int, C.c_int, C.c_double, P(_VP)])
     _sig(L, "lqt_strands_destroy", [_VP])
     _sig(L, "lqt_strands_run", [_VP])
+    _sig(L, "lqt_strands_run_async", [_VP, _VP])
     _sig(L, "lqt_strands_results", [_VP, _VP])
     _sig(L, "lqt_strands_cell_results", [_VP, C.c_int, _VP, _VP, _VP])
     _sig(L, "lqt_strands_ecg", [_VP, C.c_int, _VP, P(C.c_double), P(C.c_int64)])
@@ -176,6 +179,8 @@ def lib():
     _sig(L, "lqt_tissue_autotune", [_VP, C.c_int, P(C.c_int), _VP])
     _sig(L, "lqt_vw_search", [P(StrandDesc), _VP, _VP, P(Params), C.c_int, C.c_double, P(VWDesc), P(VWResult),
                               _VP, _VP, C.c_int64])
+    _sig(L, "lqt_geometry_read_file", [C.c_char_p, C.c_int, C.c_int, C.c_int, _VP, _VP, P(C.c_int64)])
+    _sig(L, "lqt_stimulus_read_file", [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int64, _VP, _VP, P(C.c_int64)])
     _sig(L, "lqt_mlp_create", [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, P(_VP)])
     _sig(L, "lqt_mlp_destroy", [_VP])
     _sig(L, "lqt_mlp_get_params", [_VP] + [_VP] * 6)

@@ -301,6 +301,15 @@ lqt_status lqt_cells_step(lqt_cells* c, int nsteps, const double* istim, const d
     return LQT_OK;
 }
 
+lqt_status lqt_cells_step_async(lqt_cells* c, int nsteps, const double* istim, const double* dvgap, void* stream) {
+    LQT_CHECK(c && nsteps >= 0, LQT_ERR_ARG, "lqt_cells_step_async: bad argument");
+    if (nsteps == 0) return LQT_OK;
+    launch_cells_step(c->fp32, c->n, c->r, c->d, c->pitch, c->v, c->cls, c->pidx, c->consts, nsteps, c->dt, istim,
+                      dvgap, stream ? (hipStream_t)stream : c->stream);
+    LQT_HIP(hipGetLastError());
+    return LQT_OK;
+}
+
 lqt_status lqt_cells_get_state(lqt_cells* c, double* out) {
     LQT_CHECK(c && out, LQT_ERR_ARG, "lqt_cells_get_state: bad argument");
     const size_t rs = rsize(c);

@@ -511,44 +511,72 @@ lqt_status lqt_tissue_set_stimulus(lqt_tissue* t, const lqt_stim_protocol* p, in
     return LQT_OK;
 }
 
-lqt_status lqt_tissue_set_stimulus_file(lqt_tissue* t, const lqt_stim_protocol* p, const char* path) {
-    LQT_CHECK(t && p && path, LQT_ERR_ARG, "lqt_tissue_set_stimulus_file: null argument");
+// The stimulation profile's sites inside an nx x ny x nz grid, in file
+// order, with their offsets in ms (host only).
+static lqt_status read_stimulus(const char* path, int nx, int ny, int nz, std::vector<int64_t>& cells,
+                                std::vector<float>& offs) {
     FILE* f = fopen(path, "r");
     LQT_CHECK(f, LQT_ERR_IO, "cannot open %s", path);
-    std::vector<int64_t> cells;
-    std::vector<float> offs;
     int x, y, z;
     float a4, a5, a6, tact;
     char tag[64];
     // "x y z f f f t_act str" (GPU3D.cu:1700); t_act in seconds
     while (fscanf(f, "%d %d %d %f %f %f %f %63s", &x, &y, &z, &a4, &a5, &a6, &tact, tag) >= 7) {
-        if (x < 0 || x >= t->nx || y < 0 || y >= t->ny || z < 0 || z >= t->nz) continue;
-        cells.push_back(((int64_t)x * t->ny + y) * t->nz + z);
+        if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) continue;
+        cells.push_back(((int64_t)x * ny + y) * nz + z);
         offs.push_back(1000.0f * tact);
     }
     fclose(f);
+    return LQT_OK;
+}
+
+lqt_status lqt_stimulus_read_file(const char* path, int nx, int ny, int nz, int64_t cap, int64_t* cells,
+                                  float* offsets_ms, int64_t* n) {
+    LQT_CHECK(path && n && nx > 0 && ny > 0 && nz > 0 && cap >= 0, LQT_ERR_ARG, "lqt_stimulus_read_file: bad argument");
+    std::vector<int64_t> c;
+    std::vector<float> o;
+    lqt_status s = read_stimulus(path, nx, ny, nz, c, o);
+    if (s) return s;
+    *n = (int64_t)c.size();
+    if (cells || offsets_ms) {
+        LQT_CHECK((int64_t)c.size() <= cap, LQT_ERR_ARG, "lqt_stimulus_read_file: %s holds %lld sites, cap is %lld",
+                  path, (long long)c.size(), (long long)cap);
+        for (size_t i = 0; i < c.size(); ++i) {
+            if (cells) cells[i] = c[i];
+            if (offsets_ms) offsets_ms[i] = o[i];
+        }
+    }
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_set_stimulus_file(lqt_tissue* t, const lqt_stim_protocol* p, const char* path) {
+    LQT_CHECK(t && p && path, LQT_ERR_ARG, "lqt_tissue_set_stimulus_file: null argument");
+    std::vector<int64_t> cells;
+    std::vector<float> offs;
+    lqt_status s = read_stimulus(path, t->nx, t->ny, t->nz, cells, offs);
+    if (s) return s;
     return lqt_tissue_set_stimulus(t, p, (int64_t)cells.size(), cells.data(), offs.data(), 0, nullptr);
 }
 
-lqt_status lqt_tissue_step_async(lqt_tissue* t, int64_t nsteps, void* stream) {
-    LQT_CHECK(t && nsteps >= 0, LQT_ERR_ARG, "lqt_tissue_step: bad argument");
-    hipStream_t s = stream ? (hipStream_t)stream : t->stream;
-    // every 16th step carries a pair of timing events (at most 128 pairs)
-    const int64_t ntime = std::min<int64_t>((nsteps + 15) / 16, 128);
-    while ((int64_t)t->ev.size() < 2 * ntime) {
-        hipEvent_t e;
-        LQT_HIP(hipEventCreate(&e));
-        t->ev.push_back(e);
+// nsteps steps enqueued on stream s.  With `timed`, one pair of events on s
+// brackets all of them (the mean device time per step is read after the
+// caller's synchronisation); without it nothing waits for the device.
+static lqt_status step_enqueue(lqt_tissue* t, int64_t nsteps, hipStream_t s, bool timed) {
+    timed = timed && nsteps > 0;
+    if (timed) {
+        while (t->ev.size() < 2) {
+            hipEvent_t e;
+            LQT_HIP(hipEventCreate(&e));
+            t->ev.push_back(e);
+        }
+        LQT_HIP(hipEventRecord(t->ev[0], s));
     }
     for (int64_t i = 0; i < nsteps; ++i) {
-        const int64_t ti = i / 16;
-        const bool timed = (i % 16) == 0 && ti < ntime;
         const double tc = t->time - floor(t->time / t->proto.bcl) * t->proto.bcl;
         const int stim_on = t->nsites > 0 && tc >= t->off_min && tc < t->off_max + t->proto.duration &&
                             (t->proto.window <= 0 || tc < t->proto.window);
         const int s2_on = t->ns2 > 0 && t->proto.s2_duration > 0 && t->time >= t->proto.s2_time &&
                           t->time < t->proto.s2_time + t->proto.s2_duration;
-        if (timed) LQT_HIP(hipEventRecord(t->ev[2 * ti], s));
         if (t->fp32) {
             TissueArgs<float> a;
             fill_args(t, a);
@@ -564,7 +592,6 @@ lqt_status lqt_tissue_step_async(lqt_tissue* t, int64_t nsteps, void* stream) {
             a.s2_on = s2_on;
             launch_tissue_step(false, t->aniso, t->nz == 1, t->variant, &a, t->nvox, s);
         }
-        if (timed) LQT_HIP(hipEventRecord(t->ev[2 * ti + 1], s));
         if (t->trk_on && t->time >= t->trk_from && t->time < t->trk_to)
             launch_track(t->fp32, t->nvox, t->vpos, t->grid[t->cur ^ 1], t->time, t->track, s);
         LQT_HIP(hipGetLastError());
@@ -572,24 +599,26 @@ lqt_status lqt_tissue_step_async(lqt_tissue* t, int64_t nsteps, void* stream) {
         t->time += t->dt;  // GPU3D.cu:1840 double accumulator
         t->step += 1;
     }
-    if (ntime > 0) {
-        LQT_HIP(hipStreamSynchronize(s));
-        double a = 0;
-        for (int64_t i = 0; i < ntime; ++i) {
-            float m1 = 0;
-            LQT_HIP(hipEventElapsedTime(&m1, t->ev[2 * i], t->ev[2 * i + 1]));
-            a += m1;
-        }
-        t->us_step = 1000.0 * a / ntime;
-        t->us_ghost = 0.0;  // the step kernel refreshes the ghosts itself
-    }
+    if (timed) LQT_HIP(hipEventRecord(t->ev[1], s));
     return LQT_OK;
 }
 
+lqt_status lqt_tissue_step_async(lqt_tissue* t, int64_t nsteps, void* stream) {
+    LQT_CHECK(t && nsteps >= 0, LQT_ERR_ARG, "lqt_tissue_step_async: bad argument");
+    return step_enqueue(t, nsteps, stream ? (hipStream_t)stream : t->stream, false);
+}
+
 lqt_status lqt_tissue_step(lqt_tissue* t, int64_t nsteps) {
-    lqt_status s = lqt_tissue_step_async(t, nsteps, nullptr);
+    LQT_CHECK(t && nsteps >= 0, LQT_ERR_ARG, "lqt_tissue_step: bad argument");
+    lqt_status s = step_enqueue(t, nsteps, t->stream, true);
     if (s) return s;
     LQT_HIP(hipStreamSynchronize(t->stream));
+    if (nsteps > 0) {
+        float ms = 0;
+        LQT_HIP(hipEventElapsedTime(&ms, t->ev[0], t->ev[1]));
+        t->us_step = 1000.0 * ms / nsteps;
+        t->us_ghost = 0.0;  // the step kernel refreshes the ghosts itself
+    }
     unsigned long long g[3];
     LQT_HIP(hipMemcpy(g, t->guard, sizeof(g), hipMemcpyDeviceToHost));
     if (g[2]) return set_error(LQT_ERR_RANGE, "non-finite V in %llu voxel-steps (t = %.3f ms)", g[2], t->time);
@@ -888,15 +917,15 @@ static uint8_t label20_to_celltype(int l) {
     return LQT_ENDO;
 }
 
-lqt_status lqt_tissue_create_from_file(const char* path, int nx, int ny, int nz, double dx, double d_fibre,
-                                       double d_cross, const lqt_params* params, lqt_precision prec, double dt,
-                                       lqt_tissue** out) {
-    LQT_CHECK(path && nx > 0 && ny > 0 && nz > 0, LQT_ERR_ARG, "lqt_tissue_create_from_file: bad argument");
+lqt_status lqt_geometry_read_file(const char* path, int nx, int ny, int nz, uint8_t* label, float* fibre,
+                                  int64_t* nvox) {
+    LQT_CHECK(path && label && nx > 0 && ny > 0 && nz > 0, LQT_ERR_ARG, "lqt_geometry_read_file: bad argument");
+    const int64_t ncell = (int64_t)nx * ny * nz;
+    for (int64_t c = 0; c < ncell; ++c) label[c] = LQT_NONE;
+    if (fibre)
+        for (int64_t c = 0; c < 3 * ncell; ++c) fibre[c] = 0.0f;
     FILE* f = fopen(path, "r");
     LQT_CHECK(f, LQT_ERR_IO, "cannot open %s", path);
-    const int64_t ncell = (int64_t)nx * ny * nz;
-    std::vector<uint8_t> label(ncell, LQT_NONE);
-    std::vector<float> fib((size_t)ncell * 3, 0.0f);
     int x, y, z, l;
     float f0, f1, f2;
     int64_t nread = 0;
@@ -908,13 +937,32 @@ lqt_status lqt_tissue_create_from_file(const char* path, int nx, int ny, int nz,
         }
         const int64_t c = ((int64_t)x * ny + y) * nz + z;
         label[c] = label20_to_celltype(l);
-        fib[3 * c] = f0;
-        fib[3 * c + 1] = f1;
-        fib[3 * c + 2] = f2;
+        if (fibre) {
+            fibre[3 * c] = f0;
+            fibre[3 * c + 1] = f1;
+            fibre[3 * c + 2] = f2;
+        }
         ++nread;
     }
     fclose(f);
     LQT_CHECK(nread > 0, LQT_ERR_IO, "%s: no voxel read", path);
+    if (nvox) {
+        int64_t k = 0;
+        for (int64_t c = 0; c < ncell; ++c) k += label[c] != LQT_NONE;
+        *nvox = k;
+    }
+    return LQT_OK;
+}
+
+lqt_status lqt_tissue_create_from_file(const char* path, int nx, int ny, int nz, double dx, double d_fibre,
+                                       double d_cross, const lqt_params* params, lqt_precision prec, double dt,
+                                       lqt_tissue** out) {
+    LQT_CHECK(path && nx > 0 && ny > 0 && nz > 0, LQT_ERR_ARG, "lqt_tissue_create_from_file: bad argument");
+    const int64_t ncell = (int64_t)nx * ny * nz;
+    std::vector<uint8_t> label(ncell);
+    std::vector<float> fib((size_t)ncell * 3);
+    lqt_status s = lqt_geometry_read_file(path, nx, ny, nz, label.data(), fib.data(), nullptr);
+    if (s) return s;
     lqt_geometry g;
     g.nx = nx;
     g.ny = ny;

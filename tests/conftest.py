@@ -28,3 +28,21 @@ def oracle():
 
     O.lib()
     return O
+
+
+@pytest.fixture(scope="session")
+def hip(lqt):
+    """The HIP runtime liblqt.so links, through ctypes: device buffers and
+    streams for the tests of the entry points that take device memory or a
+    caller's stream."""
+    import ctypes as C
+
+    lqt.lib()
+    h = C.CDLL("libamdhip64.so.7")
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipFree.argtypes = [C.c_void_p]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    h.hipStreamSynchronize.argtypes = [C.c_void_p]
+    h.hipStreamDestroy.argtypes = [C.c_void_p]
+    return h
