@@ -9,10 +9,14 @@ Workload (the metric's 3D configuration): synthetic biventricular shell on a
 A step is one fused tissue step (diffusion + TPORd update of every voxel,
 plus the ghost refresh) with all state resident in HBM.  The headline value
 is FP32 cell-steps/s; the plain FP64 path is timed on the same geometry and
-steps, and a strand check gives the APD90 and activation-time error of FP32
-against FP64.
+steps, and the per-voxel activation and APD90 maps of the first beat in
+both precisions give the accuracy gate ("gate").  Further legs: the
+512-strand batched sweep (strands/s), the 2D annulus sheet, and the
+surrogate-steered dose sweep against the plain batched sweep of the same
+grid ("surrogate"); the FP64 oracle on the host cores is timed beside them.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-strands]
+                    [--no-sheet] [--no-surrogate] [--no-accuracy3d]
 """
 from __future__ import annotations
 
@@ -26,6 +30,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# FP32 mixed fused step: 2 x (V 4 + 28 gates x 4 + 4 fast concentrations x 4
+# + 5 slow x 8) state + label 1 + fibre 12 + compact index 4
+ALG_BYTES_PER_VOXEL_STEP = 361
 # rocprofv3 --pmc summary of the FP32 tissue kernel on this workload
 PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "tissue_step_ventricle3d.md")
 
@@ -60,7 +67,7 @@ def accuracy_3d(A, geo, t_end=1000.0):
     [0, t_end) ms), compared voxel by voxel."""
     import numpy as np
 
-    maps = {}
+    maps, vend = {}, {}
     for name, prec in (("fp32", A.FP32), ("fp64", A.FP64)):
         T = A.Tissue(geo.label, geo.fibre, geo.dx, 0.154, 0.0385, A.preset("config3"), prec, 0.02)
         T.set_stimulus(geo.stim_sites, geo.stim_offsets, bcl=1000.0, duration=3.0, strength=-52.0)
@@ -71,8 +78,10 @@ def accuracy_3d(A, geo, t_end=1000.0):
             n += 1
         T.step(n)
         maps[name] = T.track_results()
+        vend[name] = T.v()
         T.close()
     (a32, p32), (a64, p64) = maps["fp32"], maps["fp64"]
+    dv = np.abs(vend["fp32"] - vend["fp64"])
     both = (a32 >= 0) & (a64 >= 0)
     rep = both & (p32 >= 0) & (p64 >= 0)
     d_act = np.abs(a32[both] - a64[both])
@@ -87,7 +96,22 @@ def accuracy_3d(A, geo, t_end=1000.0):
                 d_act_steps_over_1=int((d_act > 0.02 + 1e-9).sum()),
                 d_apd90_ms_max=float(d_apd.max()) if d_apd.size else None,
                 d_apd90_ms_p99=float(np.percentile(d_apd, 99)) if d_apd.size else None,
-                d_apd90_over_1ms=int((d_apd > 1.0).sum()))
+                d_apd90_over_1ms=int((d_apd > 1.0).sum()),
+                v_linf_mv_at_window_end=float(dv.max()),
+                v_abs_diff_mv_p999_at_window_end=float(np.percentile(dv, 99.9)))
+
+
+def gate_summary(vb):
+    """The metric's accuracy gate (activation within one step, APD90 within
+    1 ms) over the ventricle's first beat, from accuracy_3d's counts: met
+    everywhere or not, and where not, how many voxels fail it."""
+    over, one = vb["d_apd90_over_1ms"], vb["repolarised_one_only"]
+    act_ok = vb["d_act_steps_over_1"] == 0 and vb["activated_one_only"] == 0
+    return dict(criterion="FP32 vs FP64 per voxel over the first 1000 ms: activation <= 1 dt, dAPD90 <= 1 ms",
+                met_everywhere=bool(act_ok and over == 0 and one == 0), activation_within_1dt_everywhere=bool(act_ok),
+                apd90_over_1ms=over, repolarised_one_precision_only=one, repolarised_both=vb["repolarised"],
+                exception_fraction_of_repolarised=(over + one) / max(1, vb["repolarised"] + one),
+                voxels=vb["voxels"])
 
 
 def cpu_baseline(geo, steps, threads):
@@ -149,7 +173,8 @@ def measured_traffic(shape):
 
 def strand_accuracy(A):
     """FP32 vs FP64 on the transmural strand of OneD.cc (100 cells, ISO off,
-    2 beats): worst activation-time and APD90 difference over all cells."""
+    2 beats): worst activation-time and APD90 difference over all cells and
+    the QT difference (a strand check, not the ventricle's gate)."""
     import numpy as np
 
     spec = [A.StrandSpec(A.transmural(100), A.preset("config1"), bcl=1000.0, nbeats=2)]
@@ -161,7 +186,8 @@ def strand_accuracy(A):
         st.close()
     a64, p64, _, q64 = res[A.FP64]
     a32, p32, _, q32 = res[A.FP32]
-    return dict(d_act_ms=float(np.max(np.abs(a32 - a64))), d_apd90_ms=float(np.max(np.abs(p32 - p64))),
+    return dict(workload="transmural strand, 100 cells, config-1 parameters (ISO off), 2 beats at BCL 1000 ms",
+                d_act_ms=float(np.max(np.abs(a32 - a64))), d_apd90_ms=float(np.max(np.abs(p32 - p64))),
                 d_qt_ms=float(abs(q32 - q64)))
 
 
@@ -269,6 +295,22 @@ def strands_throughput(A, nstr, nbeats, seed=0):
     return out
 
 
+def surrogate_leg(args):
+    """The surrogate-steered dose sweep (config 5): training strands from the
+    batched solver, the MLP trained on the GPU, the plain batched sweep of
+    the 100 x 100 x 2 grid once and one steered sweep per QT threshold (the
+    50th, 75th, 90th and 97.5th percentiles of the training QT), each
+    compared point by point with the plain sweep."""
+    from lqt1_amd import sweep as SW
+
+    a = SW.parser().parse_args(["--train", str(args.surrogate_train), "--grid", str(args.surrogate_grid),
+                                "--beats", "10", "--plain", "--seed", str(args.seed)])
+    t0 = time.perf_counter()
+    out = SW.run(a, log=log)
+    out["wall_s"] = time.perf_counter() - t0
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,6 +325,9 @@ def main():
     ap.add_argument("--no-accuracy3d", action="store_true",
                     help="skip the per-voxel FP32 vs FP64 activation / APD90 comparison over the first beat")
     ap.add_argument("--strand-beats", type=int, default=10, help="beats per strand of the 512-strand sweep")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate-steered sweep leg")
+    ap.add_argument("--surrogate-train", type=int, default=20000, help="training strands of the surrogate leg")
+    ap.add_argument("--surrogate-grid", type=int, default=100, help="IKs and dose points of the swept grid")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--profile", choices=["fp32", "fp64", "copy", "cells", "strands", "strand-latency", "sheet"],
                     default=None,
@@ -323,14 +368,18 @@ def main():
         # one strand alone: its workgroup is the only one on its CU, so the
         # step time is the latency of one cell update at 1 or 2 wavefronts
         # per SIMD (the critical path of a sweep)
+        # (layout 0: one lane per cell; layout 3: strands up to 256 cells
+        # split over two lanes per cell)
         p = A.preset("cpu_default", iks_scale=0.5, iso=1, suprep=1, dose_fmol=100.0)
-        for n in (64, 128, 256, 320, 400, 512):
-            st = A.Strands([A.StrandSpec(A.transmural(n, 0.3, 0.3), p, bcl=1000.0, nbeats=1)], A.FP32, 0.02)
-            st.run()
-            st.run()
-            us = st.last_ms() * 1e3 / (1000.0 / 0.02)
-            st.close()
-            log(f"strand-latency: {n} cells, {(n + 63) // 64} wavefronts: {us:.2f} us per step")
+        for n in (64, 100, 128, 192, 256, 320, 400, 512):
+            for mode in (0, 3):
+                st = A.Strands([A.StrandSpec(A.transmural(n, 0.3, 0.3), p, bcl=1000.0, nbeats=1)], A.FP32, 0.02)
+                st.set_mode(mode)
+                st.run()
+                st.run()
+                us = st.last_ms() * 1e3 / (1000.0 / 0.02)
+                st.close()
+                log(f"strand-latency: {n} cells, {(n + 63) // 64} wavefronts, layout {mode}: {us:.2f} us per step")
         return
     if args.profile == "cells":
         # the ionic update alone on the ventricle's voxel count and type mix
@@ -368,30 +417,42 @@ def main():
     nvox = r32["nvox"]
     v32 = nvox * args.steps / r32["wall"]
     v64 = nvox * k64 / r64["wall"]
-    acc = dict(v_linf_mv_fp32_vs_fp64=float(np.max(np.abs(r32["v"] - r64["v"]))) if k64 == args.steps else None)
+    acc = {}
     try:
-        acc.update(strand_accuracy(A))
+        acc["strand_config1"] = strand_accuracy(A)
     except Exception as e:  # reported, not hidden
         acc["strand_error"] = str(e)
+    gate = None
     if not args.no_accuracy3d:
         try:
             t0 = time.perf_counter()
             acc["ventricle_first_beat"] = accuracy_3d(A, geo)
+            gate = gate_summary(acc["ventricle_first_beat"])
             log(f"3D accuracy ({time.perf_counter() - t0:.1f} s): {acc['ventricle_first_beat']}")
         except Exception as e:
             acc["ventricle_error"] = str(e)
-    achieved = r32["bytes"] / (r32["us_main"] * 1e-6) / 1e9
+    # Algorithmic bytes per launch: 361 B per voxel-step (V, 28 gates and 4
+    # fast concentrations in FP32 and 5 slow ones in FP64, read and written;
+    # label, fibre and compact index read) times the voxels of one launch.
+    # The implementation's own count adds the stored divergence of D and the
+    # ghost-cell writes (bytes_per_launch_impl).  The kernel time is the mean
+    # per launch of one pair of HIP events around the timed steps on the
+    # tissue's stream.
+    alg_bytes = ALG_BYTES_PER_VOXEL_STEP * nvox
+    achieved = alg_bytes / (r32["us_main"] * 1e-6) / 1e9
     traffic = measured_traffic(r32["shape"])
     # the practical streaming ceiling of this GPU: a plain 2 GiB copy
     copy_us = A._lib.bench_copy(2 << 30, 20)
     copy_gbs = 2 * (2 << 30) / (copy_us * 1e-6) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                traffic_over_algorithmic=round(traffic / r32["bytes"], 4) if traffic else None,
+                traffic_over_algorithmic=round(traffic / alg_bytes, 4) if traffic else None,
                 copy_gbs=round(copy_gbs, 1), frac_of_copy=round(achieved / copy_gbs, 4),
                 kernel=kernel_name(r32["shape"]),
                 kernel_us=round(r32["us_main"], 2), tune=r32["tune"],
-                bytes_per_launch=r32["bytes"])
+                bytes_per_voxel_step=ALG_BYTES_PER_VOXEL_STEP, bytes_per_launch=alg_bytes,
+                bytes_per_launch_impl=r32["bytes"],
+                achieved_impl=round(r32["bytes"] / (r32["us_main"] * 1e-6) / 1e9, 1))
 
     strands = None
     if not args.no_strands:
@@ -409,6 +470,14 @@ def main():
             log(f"sheet2d: {sheet}")
         except Exception as e:
             sheet = dict(error=str(e))
+
+    surrogate = None
+    if not args.no_surrogate:
+        try:
+            surrogate = surrogate_leg(args)
+            log(f"surrogate: {surrogate}")
+        except Exception as e:
+            surrogate = dict(error=str(e))
 
     cpu = None
     if not args.no_cpu:
@@ -445,8 +514,10 @@ def main():
                  "kernel_us": round(r64["us_main"], 2), "tune": r64["tune"]},
         "fp32_over_fp64": v32 / v64,
         "accuracy": acc,
+        "gate": gate,
         "strands": strands,
         "sheet2d": sheet,
+        "surrogate": surrogate,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

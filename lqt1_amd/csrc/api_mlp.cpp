@@ -78,34 +78,50 @@ static lqt_status ensure_batch(lqt_mlp* m, int B) {
     return LQT_OK;
 }
 
-// Forward pass of B rows already in xb/xt (m_valid real rows).
-static void forward(lqt_mlp* m, int B, int mv, bool keep_t) {
+// Buffers of one forward pass: the packed input, the hidden activations
+// (bf16, and optionally their transposes and pre-activation fp32 copies for
+// the backward pass) and the padded fp32 output.
+struct FwdBufs {
+    const __bf16* xb;
+    __bf16 *h1b, *h2b, *h1t, *h2t;  // h1t/h2t may be null
+    float *h1pre, *h2pre;           // may be null
+    float* out;
+};
+
+// Forward pass of B rows already in b.xb (m_valid real rows).
+static void forward_into(const lqt_mlp* m, const FwdBufs& b, int B, int mv, hipStream_t st) {
     const int H = m->H;
     GemmEpi e{};
     e.bias = m->b1;
     e.act = 1;
-    e.c32 = m->h1pre;
+    e.c32 = b.h1pre;
     e.ldc32 = H;
-    e.cb = m->h1b;
+    e.cb = b.h1b;
     e.ldcb = H;
-    e.cbt = keep_t ? m->h1t : nullptr;
+    e.cbt = b.h1t;
     e.ldcbt = B;
     e.n_valid = H;
     e.m_valid = mv;
-    launch_gemm_bt(m->xb, K1, m->w1b, K1, B, H, K1, e, m->stream);
+    launch_gemm_bt(b.xb, K1, m->w1b, K1, B, H, K1, e, st);
     e.bias = m->b2;
-    e.c32 = m->h2pre;
-    e.cb = m->h2b;
-    e.cbt = keep_t ? m->h2t : nullptr;
-    launch_gemm_bt(m->h1b, H, m->w2b, H, B, H, H, e, m->stream);
+    e.c32 = b.h2pre;
+    e.cb = b.h2b;
+    e.cbt = b.h2t;
+    launch_gemm_bt(b.h1b, H, m->w2b, H, B, H, H, e, st);
     GemmEpi o{};
     o.bias = m->b3;
     o.act = 0;
-    o.c32 = m->out;
+    o.c32 = b.out;
     o.ldc32 = N3;
     o.n_valid = N3;
     o.m_valid = mv;
-    launch_gemm_bt(m->h2b, H, m->w3b, H, B, N3, H, o, m->stream);
+    launch_gemm_bt(b.h2b, H, m->w3b, H, B, N3, H, o, st);
+}
+
+static void forward(lqt_mlp* m, int B, int mv, bool keep_t) {
+    const FwdBufs b{m->xb, m->h1b, m->h2b, keep_t ? m->h1t : nullptr, keep_t ? m->h2t : nullptr,
+                    m->h1pre, m->h2pre, m->out};
+    forward_into(m, b, B, mv, m->stream);
 }
 
 // Loss, backward pass and (if adam) the parameter update for the batch in
@@ -446,7 +462,14 @@ lqt_status lqt_mlp_train(lqt_mlp* m, int64_t n, const float* x, const float* y, 
     LQT_HIP(hipStreamEndCapture(m->stream, &g));
     m->graph = g;
     LQT_HIP(hipGraphInstantiate(&m->exec, m->graph, nullptr, nullptr, 0));
-    for (int64_t i = 0; i < steps; ++i) LQT_HIP(hipGraphLaunch(m->exec, m->stream));
+    // At most kInFlight replays (about 20 kernels each) are queued ahead of
+    // the device: the host waits for the stream after every kInFlight
+    // launches instead of queueing all epochs * nb replays at once.
+    constexpr int64_t kInFlight = 64;
+    for (int64_t i = 0; i < steps; ++i) {
+        LQT_HIP(hipGraphLaunch(m->exec, m->stream));
+        if ((i + 1) % kInFlight == 0) LQT_HIP(hipStreamSynchronize(m->stream));
+    }
     LQT_HIP(hipStreamSynchronize(m->stream));
     if (loss_out) LQT_HIP(hipMemcpy(loss_out, m->loss_log, sizeof(float) * steps, hipMemcpyDeviceToHost));
     return LQT_OK;

@@ -49,7 +49,8 @@ lqt_status lqt_surrogate_sweep(lqt_mlp* net, int n_iks, const double* iks, int n
                   n_dose > 0 && n_iso > 0,
               LQT_ERR_ARG, "lqt_surrogate_sweep: bad argument");
     LQT_CHECK(d->bcl > 0 && d->ncell >= 2 && d->ncell <= 512 && d->mfrac >= 0 && d->mfrac <= 1 && d->nbeats > 0 &&
-                  d->band_ms >= 0 && d->dose_max_fmol >= 0 && d->chunk >= 0 && d->autotune_steps >= 0,
+                  d->band_ms >= 0 && d->dose_max_fmol >= 0 && d->chunk >= 0 && d->autotune_steps >= 0 &&
+                  d->pca_iso >= 0,
               LQT_ERR_ARG, "lqt_surrogate_sweep: bad sweep description");
     const int64_t np = (int64_t)n_iks * n_dose * n_iso;
 
@@ -101,6 +102,7 @@ lqt_status lqt_surrogate_sweep(lqt_mlp* net, int n_iks, const double* iks, int n
             par[k].iso = iso[c];
             par[k].suprep = 1;
             par[k].dose_fmol = dose[b] * d->dose_max_fmol;
+            if (d->pca_iso > 0) par[k].pca_iso = d->pca_iso;
             transmural(L, 0.5 * (1.0 - d->mfrac), d->mfrac, &ct[(size_t)k * L]);
             lqt_strand_desc& s = desc[k];
             memset(&s, 0, sizeof(s));
@@ -134,14 +136,19 @@ lqt_status lqt_surrogate_sweep(lqt_mlp* net, int n_iks, const double* iks, int n
         if (st) return st;
         for (int k = 0; k < n; ++k) {
             const int64_t i = near[c0 + k];
-            if (res[k].qt >= 0) qt_err[i] = qt[i] - res[k].qt;
             agree += (int64_t)(ead[i] == (res[k].ead != 0));
-            qt[i] = res[k].qt;
+            if (res[k].qt >= 0) {
+                qt_err[i] = qt[i] - res[k].qt;
+                qt[i] = res[k].qt;
+            } else {
+                qt[i] = NAN;  // no T end within the beat: QT not measurable, and long
+            }
             ead[i] = res[k].ead != 0;
         }
     }
     const double t2 = now_s();
-    for (int64_t i = 0; i < np; ++i) long_qt[i] = qt[i] > d->qt_threshold;
+    // long QT: above the threshold, or not measurable (NaN) on a re-checked point
+    for (int64_t i = 0; i < np; ++i) long_qt[i] = !(qt[i] <= d->qt_threshold);
     if (stats) {
         stats->points = np;
         stats->rechecked = (int64_t)near.size();

@@ -21,6 +21,11 @@ from .strands import StrandSpec, Strands, transmural
 
 N_IN, N_REG, N_BCE = 6, 4, 1
 DOSE_MAX_FMOL = 300.0  # the fitted dose range of fit_dose_dependent.py:122
+# PCa multiplier under ISO: the beta-adrenergic setting of the 3D program
+# (GPU3D.cu:767) and of the config-3 solver the training strands come from;
+# with it, low IKs under ISO gives early afterdepolarisations (TPORd.cc:629
+# uses 1.6, under which 10-beat strands show none)
+PCA_ISO = 4.0
 
 
 class MLP:
@@ -90,8 +95,12 @@ def features(iks, iso, dose, bcl, length, mfrac):
 T_SCALE, T_SHIFT = 100.0, 350.0
 
 
-def targets(results):
-    """(QT, APD90 x3) normalised + EAD flag; mask 0 where a value is missing."""
+def targets(results, points=None):
+    """(QT, APD90 x3) normalised + EAD flag; mask 0 where a value is missing.
+    With the points given, a strand that activates but has no T end within
+    its beat (QT not measurable: the longest QTs of all) gets the beat
+    length as a censored QT target instead of a masked one, so that the
+    network learns such points as long QT."""
     y = np.zeros((len(results), N_REG + N_BCE), np.float32)
     m = np.ones_like(y)
     for i, r in enumerate(results):
@@ -101,8 +110,23 @@ def targets(results):
                 m[i, j] = 0.0
             else:
                 y[i, j] = (v - T_SHIFT) / T_SCALE
+        if points is not None and r["qt"] < 0 and r["act_first"] >= 0:
+            y[i, 0] = (points[i].bcl - T_SHIFT) / T_SCALE
+            m[i, 0] = 1.0
         y[i, N_REG] = 1.0 if r["ead"] else 0.0
     return y, m
+
+
+def qt_of(results):
+    """Strand QT per result in ms, NaN where it is not measurable (no T end
+    within the beat)."""
+    q = np.array([r["qt"] for r in results], np.float64)
+    return np.where(q < 0, np.nan, q)
+
+
+def long_qt(qt, threshold):
+    """The sweep's decision: QT above the threshold or not measurable (NaN)."""
+    return ~(np.asarray(qt) <= threshold)
 
 
 def decode(out):
@@ -125,20 +149,20 @@ class Point:
     mfrac: float
 
 
-def spec_of(p: Point, nbeats: int) -> StrandSpec:
+def spec_of(p: Point, nbeats: int, pca_iso: float = PCA_ISO) -> StrandSpec:
     par = L.preset("cpu_default", iks_scale=float(p.iks), iso=int(p.iso), suprep=1,
-                   dose_fmol=float(p.dose) * DOSE_MAX_FMOL)
+                   dose_fmol=float(p.dose) * DOSE_MAX_FMOL, pca_iso=float(pca_iso))
     endo = 0.5 * (1.0 - p.mfrac)
     return StrandSpec(transmural(int(p.length), endo, float(p.mfrac)), par, bcl=float(p.bcl), nbeats=nbeats)
 
 
-def simulate(points, nbeats=10, precision=L.FP32, chunk=4096, progress=None):
+def simulate(points, nbeats=10, precision=L.FP32, chunk=4096, progress=None, pca_iso=PCA_ISO):
     """Run every point as a strand through the batched solver (launch layout
     chosen by the autotuner per chunk); returns the per-strand results and
     the device time.  progress(done, total) is called after each chunk."""
     out, t = [], 0.0
     for i in range(0, len(points), chunk):
-        st = Strands([spec_of(p, nbeats) for p in points[i:i + chunk]], precision, 0.02)
+        st = Strands([spec_of(p, nbeats, pca_iso) for p in points[i:i + chunk]], precision, 0.02)
         t0 = time.perf_counter()
         st.autotune(2000)
         st.run()
@@ -179,7 +203,7 @@ class SweepResult:
 
 
 def sweep(mlp: MLP, iks_grid, dose_grid, iso_values=(0, 1), bcl=1000.0, length=100, mfrac=0.35,
-          qt_threshold=450.0, band_ms=10.0, p_band=(0.2, 0.8), nbeats=10, precision=L.FP32):
+          qt_threshold=450.0, band_ms=10.0, p_band=(0.2, 0.8), nbeats=10, precision=L.FP32, pca_iso=PCA_ISO):
     """Surrogate over the grid, then the solver on every point within band_ms
     of the QT threshold or with an EAD probability inside p_band
     (lqt_surrogate_sweep).  Grid order: iks outer, dose, iso inner."""
@@ -190,7 +214,7 @@ def sweep(mlp: MLP, iks_grid, dose_grid, iso_values=(0, 1), bcl=1000.0, length=1
     qt, err = np.zeros(n), np.zeros(n)
     ead, long_qt, near = np.zeros(n, np.uint8), np.zeros(n, np.uint8), np.zeros(n, np.uint8)
     d = L.SweepDesc(float(bcl), int(length), float(mfrac), int(nbeats), float(qt_threshold), float(band_ms),
-                    float(p_band[0]), float(p_band[1]), DOSE_MAX_FMOL, 4096, 2000)
+                    float(p_band[0]), float(p_band[1]), DOSE_MAX_FMOL, 4096, 2000, float(pca_iso))
     stats = L.SweepStats()
     L.check(L.lib().lqt_surrogate_sweep(mlp.h, iks.size, L.ptr(iks), dose.size, L.ptr(dose), iso.size, L.ptr(iso),
                                         C.byref(d), precision, L.ptr(qt), L.ptr(ead), L.ptr(long_qt), L.ptr(near),

@@ -34,7 +34,7 @@ def test_steered_sweep_matches_plain_sweep(lqt):
     ead_p = np.array([bool(r["ead"]) for r in plain])
     chk = sw.rechecked
     # the solver's own decision on every re-checked point
-    assert np.array_equal(sw.long_qt[chk], qt_p[chk] > thr)
+    assert np.array_equal(sw.long_qt[chk], S.long_qt(S.qt_of(plain), thr)[chk])
     assert np.array_equal(sw.ead[chk], ead_p[chk])
     # what the network gains: fewer solver runs, and agreement elsewhere
     frac = chk.mean()
@@ -45,3 +45,37 @@ def test_steered_sweep_matches_plain_sweep(lqt):
           f"steered {sw.t_surrogate + sw.t_recheck:.2f} s")
     assert frac < 0.6
     assert agree > 0.9
+
+
+def test_unmeasurable_qt_counts_as_long(lqt):
+    """A re-checked point whose strand has no T end within its beat gets
+    QT = NaN and the long-QT decision, the same rule as the plain sweep's.
+    One beat whose cycle is shorter than 50 ms + QT for some points and
+    longer for others (found by scanning cycle lengths) gives both kinds."""
+    from lqt1_amd import surrogate as S
+
+    iks = np.array([0.0, 0.5, 1.0])
+    dose = np.array([0.0, 1.0])
+    I, D, SS = np.meshgrid(iks, dose, [0.0, 1.0], indexing="ij")
+    tried = []
+    for bcl in np.arange(300.0, 541.0, 20.0):
+        grid = [S.Point(float(a), int(s), float(d), float(bcl), 80, 0.35)
+                for a, d, s in zip(I.ravel(), D.ravel(), SS.ravel())]
+        plain, _ = S.simulate(grid, nbeats=1)
+        qt_p = S.qt_of(plain)
+        tried.append((float(bcl), int(np.isnan(qt_p).sum())))
+        if np.isnan(qt_p).any() and np.isfinite(qt_p).any():
+            break
+    else:
+        pytest.fail(f"no cycle length gives strands both with and without a T end within the beat: {tried}")
+    print(f"BCL {bcl:.0f} ms: QT {np.round(qt_p, 1).tolist()}")
+    net = S.MLP(seed=2)  # untrained: the probability band below re-checks every point
+    thr = float(np.nanmedian(qt_p))
+    sw = S.sweep(net, iks, dose, (0, 1), bcl=float(bcl), length=80, mfrac=0.35, qt_threshold=thr, band_ms=10.0,
+                 p_band=(-1.0, 2.0), nbeats=1)
+    assert sw.rechecked.all()
+    assert np.array_equal(np.isnan(sw.qt), np.isnan(qt_p))
+    ok = ~np.isnan(qt_p)
+    assert np.array_equal(sw.qt[ok], qt_p[ok])
+    assert sw.long_qt[~ok].all()
+    assert np.array_equal(sw.long_qt, S.long_qt(qt_p, thr))
