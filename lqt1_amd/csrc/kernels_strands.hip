@@ -2,7 +2,7 @@
 // strands at a time from a work counter (each strand on whole wavefronts,
 // cells on lanes, state in registers for the whole run) and advances every
 // strand of the bundle with the loop of OneD.cc:315-436: Jacobi snapshot of
-// V in LDS, no-flux ends by copy, S1 on the first cells while the in-cycle
+// V in LDS (one barrier per step), no-flux ends by copy, S1 on the first cells while the in-cycle
 // time of the double time accumulator is in its window (or, while that
 // window is closed, the S2 of OneD_VW.cc:291-301), then the TPORd update.
 // The time tests arrive as step indices the host evaluated on the same
@@ -28,8 +28,9 @@ __device__ __forceinline__ R wave_sum(R x) {
 // SPLIT: every cell runs on two lanes of two wavefronts, which the
 // dispatcher places on different SIMDs: the first B/2 lanes advance the
 // gates of current group A (tp_group_a), the other B/2 the gates of group B
-// (tp_group_b), and the two swap their groups' currents through LDS (a
-// second barrier per step); both then finish the step with the same
+// (tp_group_b), and the two swap their groups' currents through LDS at the
+// step's barrier (the one that also publishes V); both then finish the step
+// with the same
 // arithmetic, so they hold the same V and concentrations bit for bit and
 // the result is that of the single-lane update.  A step's dependent chain
 // per wavefront is about 0.6 of the single-lane one.  The bundle describes
@@ -42,8 +43,8 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
     // one V segment per strand: its cells plus a mirrored end on each side
     __shared__ R vbuf[2][NH + 2 * MAXS];
     __shared__ float part[2][NW];
-    __shared__ R xa[SPLIT ? 7 : 1][SPLIT ? NH : 1];   // group A currents of each cell
-    __shared__ R xb[SPLIT ? 10 : 1][SPLIT ? NH : 1];  // group B currents of each cell
+    __shared__ R xa[2][SPLIT ? 7 : 1][SPLIT ? NH : 1];   // group A currents of each cell
+    __shared__ R xb[2][SPLIT ? 10 : 1][SPLIT ? NH : 1];  // group B currents of each cell
     __shared__ int s_next;
     const int half = SPLIT ? (int)(threadIdx.x >= (unsigned)NH) : 0;
     const int tid = (int)threadIdx.x - half * NH;  // lane within the half
@@ -124,6 +125,18 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
         int64_t next_rec = D.k_rec0, next_vrec = D.k_vrec0;
         double time = 0.0;
         int b = 0;
+        // V^0 into LDS with mirrored ends
+        if (live && out_half) {
+            vbuf[0][p] = v;
+            if (j == 0) vbuf[0][seg] = v;
+            if (j == n - 1) vbuf[0][seg + n + 1] = v;
+        }
+        // One barrier per step.  The gates and currents of step k need only
+        // the cell's own V^k; V^k of the neighbours (written into vbuf[b] at
+        // the end of step k-1) and, split, the other half's currents (xa/xb
+        // slot b) are read after the barrier, by the diffusion term and the
+        // epilogue.  vbuf, xa and xb alternate between two slots, so step
+        // k+1 writes the slot nobody reads in step k.
         for (int64_t k = 0; k < kmax; ++k, time += A.dt_d) {
             const bool run = has && k < D.nsteps;
             if (k >= s1_off) {  // past this S1 range: load the next one
@@ -131,11 +144,26 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
                 s1_on = s1i < D.s1_count ? A.s1r[2 * (D.s1_first + s1i)] : INT64_MAX;
                 s1_off = s1i < D.s1_count ? A.s1r[2 * (D.s1_first + s1i) + 1] : INT64_MAX;
             }
-            // V^n into LDS with mirrored ends
-            if (live && out_half) {
-                vbuf[b][p] = v;
-                if (j == 0) vbuf[b][seg] = v;
-                if (j == n - 1) vbuf[b][seg + n + 1] = v;
+            const bool act = run && live;
+            Pre<R> P;
+            GroupA<R> GA;
+            GroupB<R> GB;
+            if (act) {
+                P = tp_pre(s, v, A.dt);
+                if (!SPLIT || half == 0) GA = tp_group_a(s, v, c, cc, A.dt, P);
+                if (!SPLIT || half == 1) GB = tp_group_b(s, v, c, cc, A.dt, P);
+                if (SPLIT) {
+                    if (half == 0) {
+                        xa[b][0][tid] = GA.INa; xa[b][1][tid] = GA.INaL; xa[b][2][tid] = GA.Ito;
+                        xa[b][3][tid] = GA.IKs; xa[b][4][tid] = GA.INaK; xa[b][5][tid] = GA.INab;
+                        xa[b][6][tid] = GA.IKb;
+                    } else {
+                        xb[b][0][tid] = GB.ICaL; xb[b][1][tid] = GB.ICaNa; xb[b][2][tid] = GB.ICaK;
+                        xb[b][3][tid] = GB.IKr; xb[b][4][tid] = GB.IK1; xb[b][5][tid] = GB.INaCa_i;
+                        xb[b][6][tid] = GB.INaCa_ss; xb[b][7][tid] = GB.IpCa; xb[b][8][tid] = GB.ICab;
+                        xb[b][9][tid] = GB.Jrel;
+                    }
+                }
             }
             __syncthreads();
             // finish the sample whose per-wave partials were written last step
@@ -155,46 +183,28 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
                 fin_idx = pend_idx;
                 pend = 0;
             }
-            const bool act = run && live;
-            Pre<R> P;
-            GroupA<R> GA;
-            GroupB<R> GB;
-            R ist = R(0), dvgap = R(0);
             if (act) {
+                if (SPLIT) {
+                    if (half == 0) {
+                        GB.ICaL = xb[b][0][tid]; GB.ICaNa = xb[b][1][tid]; GB.ICaK = xb[b][2][tid];
+                        GB.IKr = xb[b][3][tid]; GB.IK1 = xb[b][4][tid]; GB.INaCa_i = xb[b][5][tid];
+                        GB.INaCa_ss = xb[b][6][tid]; GB.IpCa = xb[b][7][tid]; GB.ICab = xb[b][8][tid];
+                        GB.Jrel = xb[b][9][tid];
+                    } else {
+                        GA.INa = xa[b][0][tid]; GA.INaL = xa[b][1][tid]; GA.Ito = xa[b][2][tid];
+                        GA.IKs = xa[b][3][tid]; GA.INaK = xa[b][4][tid]; GA.INab = xa[b][5][tid];
+                        GA.IKb = xa[b][6][tid];
+                    }
+                }
                 const R vl = vbuf[b][p - 1], vr = vbuf[b][p + 1], vc = vbuf[b][p];
                 const R first = (j == 0 || j == n - 1) ? (vr - vl) * inv_dx : (vr - vl) * inv_2dx;
                 const R second = (vr + vl - R(2) * vc) * inv_dx2;
-                dvgap = RdD * first + RD * second;
+                const R dvgap = RdD * first + RD * second;
+                R ist = R(0);
                 if (k >= s1_on) {  // in-cycle time in the S1 window
                     if (j < D.nstim) ist = (R)D.stim_strength;
                 } else if (k >= D.k_s2a && k < D.k_s2b && s2_cell) {
                     ist = (R)D.s2_strength;
-                }
-                P = tp_pre(s, v, A.dt);
-                if (!SPLIT || half == 0) GA = tp_group_a(s, v, c, cc, A.dt, P);
-                if (!SPLIT || half == 1) GB = tp_group_b(s, v, c, cc, A.dt, P);
-                if (SPLIT) {
-                    if (half == 0) {
-                        xa[0][tid] = GA.INa; xa[1][tid] = GA.INaL; xa[2][tid] = GA.Ito; xa[3][tid] = GA.IKs;
-                        xa[4][tid] = GA.INaK; xa[5][tid] = GA.INab; xa[6][tid] = GA.IKb;
-                    } else {
-                        xb[0][tid] = GB.ICaL; xb[1][tid] = GB.ICaNa; xb[2][tid] = GB.ICaK; xb[3][tid] = GB.IKr;
-                        xb[4][tid] = GB.IK1; xb[5][tid] = GB.INaCa_i; xb[6][tid] = GB.INaCa_ss; xb[7][tid] = GB.IpCa;
-                        xb[8][tid] = GB.ICab; xb[9][tid] = GB.Jrel;
-                    }
-                }
-            }
-            if (SPLIT) __syncthreads();
-            if (act) {
-                if (SPLIT) {
-                    if (half == 0) {
-                        GB.ICaL = xb[0][tid]; GB.ICaNa = xb[1][tid]; GB.ICaK = xb[2][tid]; GB.IKr = xb[3][tid];
-                        GB.IK1 = xb[4][tid]; GB.INaCa_i = xb[5][tid]; GB.INaCa_ss = xb[6][tid]; GB.IpCa = xb[7][tid];
-                        GB.ICab = xb[8][tid]; GB.Jrel = xb[9][tid];
-                    } else {
-                        GA.INa = xa[0][tid]; GA.INaL = xa[1][tid]; GA.Ito = xa[2][tid]; GA.IKs = xa[3][tid];
-                        GA.INaK = xa[4][tid]; GA.INab = xa[5][tid]; GA.IKb = xa[6][tid];
-                    }
                 }
                 v = tp_finish<R, false>(s, v, c, cc, A.dt, ist, dvgap, P, GA, GB, nullptr);
             }
@@ -246,6 +256,12 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
                         ++nvrow;
                     }
                 }
+            }
+            // V^{k+1} into the other slot with mirrored ends
+            if (live && out_half) {
+                vbuf[b ^ 1][p] = v;
+                if (j == 0) vbuf[b ^ 1][seg] = v;
+                if (j == n - 1) vbuf[b ^ 1][seg + n + 1] = v;
             }
             b ^= 1;
         }
