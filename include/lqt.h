@@ -399,6 +399,19 @@ lqt_status lqt_mlp_set_params(lqt_mlp *m, const float *w1, const float *b1, cons
                               const float *w3, const float *b3);
 /* y[n*n_out] = network(x[n*n_in]) (raw outputs; the BCE outputs are logits). */
 lqt_status lqt_mlp_forward(lqt_mlp *m, int64_t n, const float *x, float *y);
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc): the forward pass of lqt_mlp_forward on device
+ * buffers, allocating nothing.  x[n*n_in] and y[n*n_out] are device memory
+ * on the handle's device, which must be current; the work is enqueued on
+ * `stream` (a hipStream_t, NULL for the null stream) and not waited for.
+ * The intermediate buffers live in a fixed 8 MiB device array of the
+ * library (one per GPU, held until the process ends), so n is limited to
+ * floor(8 MiB / (192 + 4 hidden) / 32) * 32 rows, less alignment: 11,904
+ * rows at hidden 128; larger n is refused with LQT_ERR_ARG.  Calls that
+ * overlap on one GPU share that array and race.  Same kernels, same bits
+ * as lqt_mlp_forward. */
+lqt_status lqt_mlp_forward_noalloc(lqt_mlp *m, int64_t n, const float *x, float *y, void *stream);
 /* Loss and parameter gradients of one batch of n rows, no update. */
 lqt_status lqt_mlp_grads(lqt_mlp *m, int n, const float *x, const float *y, const float *mask, float *loss,
                          float *gw1, float *gb1, float *gw2, float *gb2, float *gw3, float *gb3);

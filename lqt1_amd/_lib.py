@@ -186,6 +186,7 @@ def lib():
     _sig(L, "lqt_mlp_get_params", [_VP] + [_VP] * 6)
     _sig(L, "lqt_mlp_set_params", [_VP] + [_VP] * 6)
     _sig(L, "lqt_mlp_forward", [_VP, C.c_int64, _VP, _VP])
+    _sig(L, "lqt_mlp_forward_noalloc", [_VP, C.c_int64, _VP, _VP, _VP])
     _sig(L, "lqt_mlp_grads", [_VP, C.c_int, _VP, _VP, _VP, _VP] + [_VP] * 6)
     _sig(L, "lqt_mlp_train", [_VP, C.c_int64, _VP, _VP, _VP, C.c_int, C.c_int, C.c_double, C.c_uint64, _VP])
     _sig(L, "lqt_surrogate_sweep", [_VP, C.c_int, _VP, C.c_int, _VP, C.c_int, _VP, P(SweepDesc), C.c_int, _VP, _VP,

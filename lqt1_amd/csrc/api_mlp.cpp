@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 #include <random>
 #include <vector>
@@ -21,6 +22,7 @@ constexpr int N3 = 32;  // padded output width (n_out <= 32)
 
 struct lqt_mlp {
     int n_in = 0, H = 0, n_out = 0, n_reg = 0, n_bce = 0;
+    int dev = 0;  // device the handle's memory lives on
     hipStream_t stream = nullptr;
     // fp32 masters (padded), Adam moments, gradients
     float *w1 = nullptr, *b1 = nullptr, *w2 = nullptr, *b2 = nullptr, *w3 = nullptr, *b3 = nullptr;
@@ -202,6 +204,54 @@ static lqt_status upload_bf16_weights(lqt_mlp* m) {
     return put(w3, N3, H, m->w3b, m->w3t);
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of lqt_mlp_forward_noalloc, the forward pass that allocates
+// nothing: the packed bf16 input, the two bf16 hidden activations and the
+// padded fp32 output of one call live in this file-scope device array.
+// Every device the entry point runs on holds its own copy until the process
+// ends.  On the host the array's name is only a stand-in: its device address
+// on each device comes from hipGetSymbolAddress, called once per device
+// with that device current and kept in g_scratch_ptr; only that address is
+// ever given to a kernel.
+static __device__ __align__(256) unsigned char scratch[(size_t)8 << 20];
+
+namespace {
+constexpr size_t kScratchBytes = (size_t)8 << 20;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// Bytes of scratch one call of B padded rows needs at hidden width H.
+size_t scratch_need(int64_t B, int H) {
+    return align256((size_t)B * K1 * sizeof(__bf16)) + 2 * align256((size_t)B * H * sizeof(__bf16)) +
+           align256((size_t)B * N3 * sizeof(float));
+}
+
+// Largest padded row count whose buffers fit the scratch at hidden width H.
+int64_t scratch_rows(int H) {
+    int64_t B = (int64_t)(kScratchBytes / (sizeof(__bf16) * (K1 + 2 * (size_t)H) + sizeof(float) * N3)) / 32 * 32;
+    while (B > 0 && scratch_need(B, H) > kScratchBytes) B -= 32;
+    return B;
+}
+
+// Device address of the scratch array on device dev (the current device).
+lqt_status scratch_base(int dev, unsigned char** out) {
+    LQT_CHECK(dev >= 0 && dev < kMaxDevices, LQT_ERR_ARG, "scratch: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        LQT_CHECK(e == hipSuccess && p, LQT_ERR_HIP, "scratch: hipGetSymbolAddress on device %d failed: %s", dev,
+                  hipGetErrorString(e));
+        g_scratch_ptr[dev] = (unsigned char*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return LQT_OK;
+}
+}  // namespace
+
 extern "C" {
 
 lqt_status lqt_mlp_create(int n_in, int hidden, int n_reg, int n_bce, uint64_t seed, lqt_mlp** out) {
@@ -217,6 +267,7 @@ lqt_status lqt_mlp_create(int n_in, int hidden, int n_reg, int n_bce, uint64_t s
     m->n_bce = n_bce;
     m->n_out = n_reg + n_bce;
     const int H = hidden;
+    if (hipGetDevice(&m->dev) != hipSuccess) m->dev = 0;
     const size_t cnt[6] = {(size_t)H * K1, (size_t)H, (size_t)H * H, (size_t)H, (size_t)N3 * H, (size_t)N3};
     size_t o = 0;
     for (int i = 0; i < 6; ++i) {
@@ -472,6 +523,41 @@ lqt_status lqt_mlp_train(lqt_mlp* m, int64_t n, const float* x, const float* y, 
     }
     LQT_HIP(hipStreamSynchronize(m->stream));
     if (loss_out) LQT_HIP(hipMemcpy(loss_out, m->loss_log, sizeof(float) * steps, hipMemcpyDeviceToHost));
+    return LQT_OK;
+}
+
+// Forward pass on device buffers that allocates nothing: x[n*n_in] and
+// y[n*n_out] are device memory on the handle's device, which must be
+// current; the intermediate buffers are the file-scope scratch array, so
+// n is limited to scratch_rows(hidden) rows (11,904 at hidden 128).  The
+// work is enqueued on `stream` and not waited for.  Calls that overlap on
+// one device share the scratch and race.  The kernels are those of
+// lqt_mlp_forward, so the outputs are the same bits.
+lqt_status lqt_mlp_forward_noalloc(lqt_mlp* m, int64_t n, const float* x, float* y, void* stream) {
+    LQT_CHECK(m && x && y && n > 0, LQT_ERR_ARG, "lqt_mlp_forward_noalloc: bad argument");
+    const int64_t nmax = scratch_rows(m->H);
+    LQT_CHECK(n <= nmax, LQT_ERR_ARG,
+              "lqt_mlp_forward_noalloc: %lld rows exceed the scratch limit of %lld rows at hidden width %d "
+              "(%zu bytes)",
+              (long long)n, (long long)nmax, m->H, kScratchBytes);
+    int dev = -1;
+    LQT_HIP(hipGetDevice(&dev));
+    LQT_CHECK(dev == m->dev, LQT_ERR_ARG,
+              "lqt_mlp_forward_noalloc: the handle lives on device %d, make it current (device %d is)", m->dev, dev);
+    unsigned char* base = nullptr;
+    lqt_status s = scratch_base(dev, &base);
+    if (s) return s;
+    const int rows = (int)n;
+    const int B = (rows + 31) / 32 * 32;
+    __bf16* xb = (__bf16*)base;
+    __bf16* h1b = (__bf16*)(base + align256((size_t)B * K1 * sizeof(__bf16)));
+    __bf16* h2b = (__bf16*)((unsigned char*)h1b + align256((size_t)B * m->H * sizeof(__bf16)));
+    float* out = (float*)((unsigned char*)h2b + align256((size_t)B * m->H * sizeof(__bf16)));
+    hipStream_t st = (hipStream_t)stream;
+    launch_pack(x, m->n_in, rows, xb, K1, st);
+    forward_into(m, FwdBufs{xb, h1b, h2b, nullptr, nullptr, nullptr, nullptr, out}, B, rows, st);
+    launch_unpack(out, N3, rows, m->n_out, y, st);
+    LQT_HIP(hipGetLastError());
     return LQT_OK;
 }
 

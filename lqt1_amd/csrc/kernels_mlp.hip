@@ -182,6 +182,15 @@ __global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ X, 
     for (int j = 0; j < ldx; ++j) xb[(int64_t)i * ldx + j] = (__bf16)(j < n_in ? X[(int64_t)i * n_in + j] : 0.0f);
 }
 
+// Network outputs: the first n_out of the ldo padded columns of n rows.
+__global__ __launch_bounds__(256) void unpack_kernel(const float* __restrict__ out, int ldo, int n, int n_out,
+                                                     float* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n * n_out) return;
+    const int64_t r = i / n_out, c = i % n_out;
+    y[i] = out[r * ldo + c];
+}
+
 void launch_gemm_bt(const __bf16* A, int lda, const __bf16* B, int ldb, int M, int N, int K, const GemmEpi& e,
                     hipStream_t st) {
     dim3 grid(M / 32, N / 32);
@@ -218,6 +227,11 @@ void launch_gather(const float* X, int n_in, const float* Y, const float* Mk, in
 
 void launch_pack(const float* X, int n_in, int n, __bf16* xb, int ldx, hipStream_t st) {
     pack_kernel<<<(n + 255) / 256, 256, 0, st>>>(X, n_in, n, xb, ldx);
+}
+
+void launch_unpack(const float* out, int ldo, int n, int n_out, float* y, hipStream_t st) {
+    const int64_t cnt = (int64_t)n * n_out;
+    unpack_kernel<<<(int)((cnt + 255) / 256), 256, 0, st>>>(out, ldo, n, n_out, y);
 }
 
 }  // namespace lqt

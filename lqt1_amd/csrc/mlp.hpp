@@ -32,6 +32,7 @@ void launch_gather(const float* X, int n_in, const float* Y, const float* Mk, in
                    int B, int m_valid, __bf16* xb, int ldx, __bf16* xt, int ldt, float* yb, float* mb, float* loss,
                    hipStream_t st);
 void launch_pack(const float* X, int n_in, int n, __bf16* xb, int ldx, hipStream_t st);
+void launch_unpack(const float* out, int ldo, int n, int n_out, float* y, hipStream_t st);
 void launch_set_int(int* p, int value, hipStream_t st);
 
 }  // namespace lqt
