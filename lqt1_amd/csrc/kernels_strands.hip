@@ -43,8 +43,8 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
     // one V segment per strand: its cells plus a mirrored end on each side
     __shared__ R vbuf[2][NH + 2 * MAXS];
     __shared__ float part[2][NW];
-    __shared__ R xa[2][SPLIT ? 7 : 1][SPLIT ? NH : 1];   // group A currents of each cell
-    __shared__ R xb[2][SPLIT ? 10 : 1][SPLIT ? NH : 1];  // group B currents of each cell
+    __shared__ R xa[2][SPLIT ? 5 : 1][SPLIT ? NH : 1];   // group A currents of each cell
+    __shared__ R xb[2][SPLIT ? 12 : 1][SPLIT ? NH : 1];  // group B currents of each cell
     __shared__ int s_next;
     const int half = SPLIT ? (int)(threadIdx.x >= (unsigned)NH) : 0;
     const int tid = (int)threadIdx.x - half * NH;  // lane within the half
@@ -149,20 +149,22 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
             GroupA<R> GA;
             GroupB<R> GB;
             if (act) {
-                P = tp_pre(s, v, A.dt);
-                if (!SPLIT || half == 0) GA = tp_group_a(s, v, c, cc, A.dt, P);
-                if (!SPLIT || half == 1) GB = tp_group_b(s, v, c, cc, A.dt, P);
-                if (SPLIT) {
-                    if (half == 0) {
-                        xa[b][0][tid] = GA.INa; xa[b][1][tid] = GA.INaL; xa[b][2][tid] = GA.Ito;
-                        xa[b][3][tid] = GA.IKs; xa[b][4][tid] = GA.INaK; xa[b][5][tid] = GA.INab;
-                        xa[b][6][tid] = GA.IKb;
-                    } else {
-                        xb[b][0][tid] = GB.ICaL; xb[b][1][tid] = GB.ICaNa; xb[b][2][tid] = GB.ICaK;
-                        xb[b][3][tid] = GB.IKr; xb[b][4][tid] = GB.IK1; xb[b][5][tid] = GB.INaCa_i;
-                        xb[b][6][tid] = GB.INaCa_ss; xb[b][7][tid] = GB.IpCa; xb[b][8][tid] = GB.ICab;
-                        xb[b][9][tid] = GB.Jrel;
-                    }
+                if (!SPLIT) {
+                    P = tp_pre(s, v, A.dt);
+                    GA = tp_group_a(s, v, c, cc, A.dt, P);
+                    GB = tp_group_b(s, v, c, cc, A.dt, P);
+                } else if (half == 0) {
+                    P = tp_pre<R, 1>(s, v, A.dt);
+                    GA = tp_group_a(s, v, c, cc, A.dt, P);
+                    xa[b][0][tid] = GA.INa; xa[b][1][tid] = GA.INaL; xa[b][2][tid] = GA.Ito;
+                    xa[b][3][tid] = GA.IKs; xa[b][4][tid] = GA.INaK;
+                } else {
+                    P = tp_pre<R, 2>(s, v, A.dt);
+                    GB = tp_group_b(s, v, c, cc, A.dt, P);
+                    xb[b][0][tid] = GB.ICaL; xb[b][1][tid] = GB.ICaNa; xb[b][2][tid] = GB.ICaK;
+                    xb[b][3][tid] = GB.IKr; xb[b][4][tid] = GB.IK1; xb[b][5][tid] = GB.INaCa_i;
+                    xb[b][6][tid] = GB.INaCa_ss; xb[b][7][tid] = GB.IpCa; xb[b][8][tid] = GB.ICab;
+                    xb[b][9][tid] = GB.Jrel; xb[b][10][tid] = GB.INab; xb[b][11][tid] = GB.IKb;
                 }
             }
             __syncthreads();
@@ -189,11 +191,10 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
                         GB.ICaL = xb[b][0][tid]; GB.ICaNa = xb[b][1][tid]; GB.ICaK = xb[b][2][tid];
                         GB.IKr = xb[b][3][tid]; GB.IK1 = xb[b][4][tid]; GB.INaCa_i = xb[b][5][tid];
                         GB.INaCa_ss = xb[b][6][tid]; GB.IpCa = xb[b][7][tid]; GB.ICab = xb[b][8][tid];
-                        GB.Jrel = xb[b][9][tid];
+                        GB.Jrel = xb[b][9][tid]; GB.INab = xb[b][10][tid]; GB.IKb = xb[b][11][tid];
                     } else {
                         GA.INa = xa[b][0][tid]; GA.INaL = xa[b][1][tid]; GA.Ito = xa[b][2][tid];
-                        GA.IKs = xa[b][3][tid]; GA.INaK = xa[b][4][tid]; GA.INab = xa[b][5][tid];
-                        GA.IKb = xa[b][6][tid];
+                        GA.IKs = xa[b][3][tid]; GA.INaK = xa[b][4][tid];
                     }
                 }
                 const R vl = vbuf[b][p - 1], vr = vbuf[b][p + 1], vc = vbuf[b][p];
@@ -218,6 +219,9 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
                         if (!(tr.flags & 1)) {
                             tr.flags |= 1;
                             tr.vrest = vf;
+                            // still depolarised from the previous beat: it
+                            // failed to repolarise
+                            if (vf > -60.0) tr.flags |= 16;
                         }
                         if (!(tr.flags & 2)) {
                             if (vf >= -30.0) {
@@ -272,7 +276,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
                 A.apd[cell] = (tr.flags & 4) ? tr.rep - tr.act : -1.0;
                 int e = 0;
                 if (tr.flags & 8) e |= 1;
-                if ((tr.flags & 2) && !(tr.flags & 4)) e |= 2;
+                if (((tr.flags & 2) && !(tr.flags & 4)) || (tr.flags & 16)) e |= 2;
                 A.ead[cell] = e;
                 A.finv[cell] = v;
                 if (j == 0) A.vmax[2 * (int64_t)sidx] = wmax;

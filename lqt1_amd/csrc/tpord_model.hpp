@@ -215,30 +215,36 @@ struct Pre {
     R ENa, EK, EKs, CaMKb, fCaMKp, vfrt, ndt, e1, e2, r1, r2;
 };
 
-// Group A: INa, INaL, Ito, IKs, INaK and the K/Na background currents, with
-// the gates sm sh sj mL hL hLp a iF iS ap iFp iSp xs1 xs2.
+// Group A: INa, INaL, Ito, IKs and INaK, with the gates sm sh sj mL hL hLp
+// a iF iS ap iFp iSp xs1 xs2.
 template <class R>
 struct GroupA {
-    R INa, INaL, Ito, IKs, INaK, INab, IKb;
+    R INa, INaL, Ito, IKs, INaK;
 };
 
-// Group B: ICaL/ICaNa/ICaK, IKr, IK1, INaCa, IpCa, ICab and Jrel, with the
-// gates d ff fs fcaf fcas jca nca ffp fcafp xrf xrs xk1 Jrelnp Jrelp.
+// Group B: ICaL/ICaNa/ICaK, IKr, IK1, INaCa, IpCa, the background currents
+// and Jrel, with the gates d ff fs fcaf fcas jca nca ffp fcafp xrf xrs xk1
+// Jrelnp Jrelp.  The two groups take about the same issue time: 439 VALU
+// instructions, 93 of them transcendental, against 507 with 76 (a
+// transcendental instruction occupies the SIMD four times as long).
 template <class R>
 struct GroupB {
-    R ICaL, ICaNa, ICaK, IKr, IK1, INaCa_i, INaCa_ss, IpCa, ICab, Jrel;
+    R ICaL, ICaNa, ICaK, IKr, IK1, INaCa_i, INaCa_ss, IpCa, ICab, Jrel, INab, IKb;
 };
 
-template <class R>
+// WHO: 0 for both groups, 1 for group A alone (without the ICaL/ICab term
+// r2), 2 for group B alone (without ENa and EKs); what a group reads is
+// computed the same way in every form.
+template <class R, int WHO = 0>
 __device__ __forceinline__ Pre<R> tp_pre(const CellState<R>& s, R v, R dt) {
     using M = Math<R>;
     const R nai = R(s.nai), ki = R(s.ki), cansr = R(s.cansr), cajsr = R(s.cajsr), CaMKt = R(s.CaMKt);
     const R nass = s.nass, kss = s.kss, cai = s.cai, cass = s.cass;
 
     // reversal potentials (:425-427)
-    const R ENa = R(k::RTF) * M::log(R(k::nao) * M::rcp(nai));
+    const R ENa = WHO == 2 ? R(0) : R(k::RTF) * M::log(R(k::nao) * M::rcp(nai));
     const R EK = R(k::RTF) * M::log(R(k::ko) * M::rcp(ki));
-    const R EKs = R(k::RTF) * M::log(R(k::ko + 0.01833 * k::nao) * M::rcp(ki + R(0.01833) * nai));
+    const R EKs = WHO == 2 ? R(0) : R(k::RTF) * M::log(R(k::ko + 0.01833 * k::nao) * M::rcp(ki + R(0.01833) * nai));
 
     // CaMK (:430-431); the same CaMKb is used again at :939-940
     const R CaMKb = R(k::CaMKo) * (R(1) - CaMKt) * M::rcp(R(1) + R(k::KmCaM) * M::rcp(cass));
@@ -250,7 +256,7 @@ __device__ __forceinline__ Pre<R> tp_pre(const CellState<R>& s, R v, R dt) {
     const R e1 = M::ex(vfrt);
     const R e2 = e1 * e1;
     const R r1 = M::x_over_expm1(vfrt, e1);            // vfrt/(exp(vfrt)-1)
-    const R r2 = M::x_over_expm1(R(2) * vfrt, e2);     // 2vfrt/(exp(2vfrt)-1)
+    const R r2 = WHO == 1 ? R(0) : M::x_over_expm1(R(2) * vfrt, e2);  // 2vfrt/(exp(2vfrt)-1)
     return Pre<R>{nai, ki, cansr, cajsr, CaMKt, nass, kss, cai, cass, ENa, EK, EKs, CaMKb, fCaMKp, vfrt, ndt,
                   e1, e2, r1, r2};
 }
@@ -393,12 +399,7 @@ __device__ __forceinline__ GroupA<R> tp_group_a(CellState<R>& s, R v, int cls, c
         INaK = Pnak * (JnakNa + JnakK);
     }
 
-    // background (:908-923)
-    const R xkb = M::rcp(R(1) + M::exl(v, -1.0 / 18.34, 14.48 / 18.34));
-    const R GKb = epi ? R(0.003 * 0.6) : R(0.003);
-    const R IKb = GKb * xkb * (v - EK);
-    const R INab = R(3.75e-10 * k::F) * r1 * (nai * e1 - R(k::nao));
-    return GroupA<R>{INa, INaL, Ito, IKs, INaK, INab, IKb};
+    return GroupA<R>{INa, INaL, Ito, IKs, INaK};
 }
 
 template <class R>
@@ -510,6 +511,10 @@ __device__ __forceinline__ GroupB<R> tp_group_b(CellState<R>& s, R v, int cls, c
     }
 
     // background (:908-923)
+    const R xkb = M::rcp(R(1) + M::exl(v, -1.0 / 18.34, 14.48 / 18.34));
+    const R GKb = epi ? R(0.003 * 0.6) : R(0.003);
+    const R IKb = GKb * xkb * (v - EK);
+    const R INab = R(3.75e-10 * k::F) * r1 * (nai * e1 - R(k::nao));
     const R ICab = R(2.5e-8 * 2.0 * k::F) * r2 * (cai * e2 - R(0.341 * k::cao));
     const R IpCa = R(0.0005) * cai * M::rcp(R(0.0005) + cai);
     // Jrel (:946-979), RL gates Jrelnp/Jrelp driven by ICaL
@@ -528,7 +533,7 @@ __device__ __forceinline__ GroupB<R> tp_group_b(CellState<R>& s, R v, int cls, c
     tau_relp = tau_relp < R(0.005) ? R(0.005) : tau_relp;
     s.Jrelp = Jrel_infp - (Jrel_infp - s.Jrelp) * M::decay(ndt, M::rcp(tau_relp));
     const R Jrel = (R(1) - fCaMKp) * s.Jrelnp + fCaMKp * s.Jrelp;
-    return GroupB<R>{ICaL, ICaNa, ICaK, IKr, IK1, INaCa_i, INaCa_ss, IpCa, ICab, Jrel};
+    return GroupB<R>{ICaL, ICaNa, ICaK, IKr, IK1, INaCa_i, INaCa_ss, IpCa, ICab, Jrel, INab, IKb};
 }
 
 template <class R, bool WANT_DIAG>
@@ -544,9 +549,9 @@ __device__ __forceinline__ R tp_finish(CellState<R>& s, R v, int cls, const Cons
     (void)epi; (void)mid; (void)nai; (void)ki; (void)cansr; (void)cajsr; (void)CaMKt; (void)nass; (void)kss;
     (void)cai; (void)cass; (void)ENa; (void)EK; (void)EKs; (void)CaMKb; (void)fCaMKp; (void)vfrt; (void)ndt;
     (void)e1; (void)e2; (void)r1; (void)r2;
-    const R INa = GA.INa, INaL = GA.INaL, Ito = GA.Ito, IKs = GA.IKs, INaK = GA.INaK, INab = GA.INab, IKb = GA.IKb;
+    const R INa = GA.INa, INaL = GA.INaL, Ito = GA.Ito, IKs = GA.IKs, INaK = GA.INaK;
     const R ICaL = GB.ICaL, ICaNa = GB.ICaNa, ICaK = GB.ICaK, IKr = GB.IKr, IK1 = GB.IK1, INaCa_i = GB.INaCa_i,
-            INaCa_ss = GB.INaCa_ss, IpCa = GB.IpCa, ICab = GB.ICab, Jrel = GB.Jrel;
+            INaCa_ss = GB.INaCa_ss, IpCa = GB.IpCa, ICab = GB.ICab, Jrel = GB.Jrel, INab = GB.INab, IKb = GB.IKb;
 
     // ---- voltage (:927-935)
     const R INaCa = INaCa_i + INaCa_ss;
