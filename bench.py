@@ -240,10 +240,63 @@ def sheet_run(A, sim_ms=2000.0, compare_ms=250.0, seed=0):
                          guard=T.v_guard().tolist())
         nvox = T.nvox
         T.close()
+    try:
+        acc_s2 = sheet_accuracy(A, geo, ci, sim_ms, dt)
+    except Exception as e:  # reported, not hidden
+        acc_s2 = dict(error=str(e))
     return dict(workload=f"2D annulus 600x600 @0.15 mm, isotropic D 0.154, S1 BCL 1000 ms + S2 at {ci:.0f} ms "
                          f"over a quadrant, {sim_ms:.0f} ms", voxels=nvox, steps=n_all, **out,
+                accuracy_after_s2=acc_s2,
                 fp32_over_fp64=out["fp32"]["cell_steps_per_s"] / out["fp64"]["cell_steps_per_s"],
                 v_linf_mv_fp32_vs_fp64_at_ms=[compare_ms, float(np.max(np.abs(vcmp["fp32"] - vcmp["fp64"])))])
+
+
+def _map_diff(a32, p32, a64, p64):
+    """FP32 against FP64 activation / APD90 maps of one window."""
+    import numpy as np
+
+    both = (a32 >= 0) & (a64 >= 0)
+    rep = both & (p32 >= 0) & (p64 >= 0)
+    da = np.abs(a32[both] - a64[both])
+    dp = np.abs(p32[rep] - p64[rep])
+    return dict(activated=int(both.sum()), activated_one_only=int(((a32 >= 0) != (a64 >= 0)).sum()),
+                repolarised=int(rep.sum()), repolarised_one_only=int((both & ((p32 >= 0) != (p64 >= 0))).sum()),
+                d_act_ms_max=float(da.max()) if da.size else None,
+                d_act_over_1dt=int((da > 0.02 + 1e-9).sum()),
+                d_apd90_ms_p99=float(np.percentile(dp, 99)) if dp.size else None,
+                d_apd90_ms_max=float(dp.max()) if dp.size else None,
+                d_apd90_over_1ms=int((dp > 1.0).sum()))
+
+
+def sheet_accuracy(A, geo, ci, sim_ms=2000.0, dt=0.02):
+    """FP32 against FP64 on the 2D annulus after the S2: activation and
+    APD90 maps of the tissue trackers over [S2, 1000 ms) (the S2-induced
+    wave and re-entry) and over [1000, 2000) ms (the second S1 beat), from
+    identical inputs."""
+    maps = {}
+    for name, prec in (("fp32", A.FP32), ("fp64", A.FP64)):
+        T = A.Tissue(geo.label, None, geo.dx, 0.154, 0.154, A.preset("config1"), prec, dt)
+        T.set_stimulus(geo.stim_sites, None, bcl=1000.0, duration=3.0, strength=-52.0, s2_sites=geo.s2_sites,
+                       s2_time=ci, s2_duration=3.0, s2_strength=-104.0)
+        out = []
+        t, n = 0.0, 0
+        for lo, hi in ((ci, 1000.0), (1000.0, sim_ms)):
+            k = 0
+            while t < lo:  # the tissue clock's own accumulator
+                t += dt
+                k += 1
+            T.step(k)
+            T.track(lo, hi)
+            k = 0
+            while t < hi:
+                t += dt
+                k += 1
+            T.step(k)
+            out.append(T.track_results())
+        maps[name] = out
+        T.close()
+    return {f"window_{lo:.0f}_{hi:.0f}_ms": _map_diff(*maps["fp32"][w], *maps["fp64"][w])
+            for w, (lo, hi) in enumerate(((ci, 1000.0), (1000.0, sim_ms)))}
 
 
 def strands_cpu(specs, nsample, threads):

@@ -283,7 +283,9 @@ typedef struct {
 typedef struct {
     double qt;                 /* ms, from the pseudo-ECG                   */
     double apd90[3];           /* ms, mean over ENDO, MCELL, EPI cells      */
-    int ead;                   /* 1 if any cell shows an EAD or fails to repolarise */
+    int ead;                   /* 1 if any cell shows an EAD or fails to repolarise
+                                  (within the beat, or from the beat before: V above
+                                  -60 mV at the measured beat's first step) */
     double act_first, act_last;/* ms, earliest/latest -30 mV crossing      */
     double cv;                 /* m/s between cells at 10% and 90% of length */
     int64_t steps;
@@ -440,7 +442,9 @@ typedef struct {
     double p_lo, p_hi;
     double dose_max_fmol;  /* dose 1 = this many fmol                     */
     int chunk;             /* strands per solver launch; 0 = 4096         */
-    int autotune_steps;    /* launch-layout trial steps per chunk; 0 = 2000 */
+    int autotune_steps;    /* launch-layout trial steps per chunk; 0 = 2000 (a chunk
+                              of at most one strand of <= 128 cells per CU runs
+                              the split layout without trials) */
     double pca_iso;        /* PCa multiplier under ISO (TPORd.cc:629 1.6; GPU3D.cu:767 4) */
 } lqt_sweep_desc;
 
@@ -455,8 +459,8 @@ typedef struct {
  * Per point: qt (ms), ead and long_qt decisions, rechecked flag, and
  * qt_err = predicted - solver QT on re-checked points (NaN elsewhere).  A
  * re-checked point whose strand has no T end within the beat (QT not
- * measurable, lqt_strand_result.qt = -1) gets qt = NaN and long_qt = 1:
- * long_qt = !(qt <= qt_threshold). */
+ * measurable, lqt_strand_result.qt = -1) gets qt = NaN; an EAD counts as
+ * long QT: long_qt = ead || !(qt <= qt_threshold). */
 lqt_status lqt_surrogate_sweep(lqt_mlp *net, int n_iks, const double *iks, int n_dose, const double *dose,
                                int n_iso, const int *iso, const lqt_sweep_desc *desc, lqt_precision prec,
                                double *qt, uint8_t *ead, uint8_t *long_qt, uint8_t *rechecked, double *qt_err,

@@ -85,6 +85,9 @@ lqt_status lqt_surrogate_sweep(lqt_mlp* net, int n_iks, const double* iks, int n
 
     // 2. the solver on the points near a decision threshold
     const int chunk = d->chunk > 0 ? d->chunk : 4096;
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        ncu = 0;
     const int trial = d->autotune_steps > 0 ? d->autotune_steps : 2000;
     const int L = d->ncell;
     int64_t agree = 0;
@@ -129,7 +132,12 @@ lqt_status lqt_surrogate_sweep(lqt_mlp* net, int n_iks, const double* iks, int n
         if ((st = lqt_strands_create(n, desc.data(), ct.data(), nullptr, par.data(), n, prec, 0.02, &sh))) return st;
         std::vector<lqt_strand_result> res(n);
         int mode = 0;
-        st = lqt_strands_autotune(sh, trial, &mode, nullptr);
+        // A batch that leaves CUs idle (at most one strand of up to 128 cells
+        // per CU) runs fastest split over two lanes per cell, one strand per
+        // CU at one wavefront per SIMD: no trial runs on the critical path.
+        // Larger batches are timed in every layout.
+        if (n <= ncu && L <= 128) st = lqt_strands_set_mode(sh, 3);
+        else st = lqt_strands_autotune(sh, trial, &mode, nullptr);
         if (!st) st = lqt_strands_run(sh);
         if (!st) st = lqt_strands_results(sh, res.data());
         lqt_strands_destroy(sh);
@@ -147,8 +155,10 @@ lqt_status lqt_surrogate_sweep(lqt_mlp* net, int n_iks, const double* iks, int n
         }
     }
     const double t2 = now_s();
-    // long QT: above the threshold, or not measurable (NaN) on a re-checked point
-    for (int64_t i = 0; i < np; ++i) long_qt[i] = !(qt[i] <= d->qt_threshold);
+    // long QT: above the threshold, not measurable (NaN, a re-checked point
+    // without a T end in its beat), or an EAD (whose T end is not a
+    // repolarisation end)
+    for (int64_t i = 0; i < np; ++i) long_qt[i] = ead[i] || !(qt[i] <= d->qt_threshold);
     if (stats) {
         stats->points = np;
         stats->rechecked = (int64_t)near.size();

@@ -43,11 +43,14 @@ struct Bundle {
 };
 
 // Per-cell beat tracker: -30 mV activation (OneD.cc:422), running peak,
-// 90 % repolarisation and the EAD test, all within beat `measure_beat`.
+// 90 % repolarisation and the EAD test, all within beat `measure_beat`; a
+// cell whose first V of the beat is above -60 mV has not repolarised from
+// the previous beat and counts as failing to repolarise.
 struct CellTrack {
     double act, rep;
     double vrest, vmax, vmin;
-    int flags;  // bit0 entered beat, bit1 activated, bit2 repolarised, bit3 EAD
+    int flags;  // bit0 entered beat, bit1 activated, bit2 repolarised, bit3 EAD,
+                // bit4 entered the beat depolarised (V > -60 mV)
 };
 
 template <class R>

@@ -95,24 +95,35 @@ def features(iks, iso, dose, bcl, length, mfrac):
 T_SCALE, T_SHIFT = 100.0, 350.0
 
 
+QT_CLIP = (150.0, 750.0)    # ms: the regression's QT target range
+APD_CLIP = (100.0, 900.0)   # ms
+
+
 def targets(results, points=None):
     """(QT, APD90 x3) normalised + EAD flag; mask 0 where a value is missing.
-    With the points given, a strand that activates but has no T end within
-    its beat (QT not measurable: the longest QTs of all) gets the beat
-    length as a censored QT target instead of a masked one, so that the
-    network learns such points as long QT."""
+
+    The QT of a strand with an EAD is masked (its pseudo-ECG's T end is not a
+    repolarisation end, and the sweep's decision for it follows the EAD
+    call).  With the points given, a strand that activates without an EAD
+    but has no T end within its beat (QT not measurable, the longest QTs of
+    all) gets the beat length as a censored target.  Targets are clipped to
+    QT_CLIP / APD_CLIP, so that a few extreme strands do not dominate the
+    squared error."""
     y = np.zeros((len(results), N_REG + N_BCE), np.float32)
     m = np.ones_like(y)
     for i, r in enumerate(results):
-        vals = [r["qt"]] + list(r["apd90"])
+        qt = r["qt"]
+        if r["ead"]:
+            qt = None
+        elif qt < 0 and points is not None and r["act_first"] >= 0:
+            qt = points[i].bcl
+        vals = [None if qt is None or qt < 0 else float(np.clip(qt, *QT_CLIP))]
+        vals += [None if v is None or v < 0 else float(np.clip(v, *APD_CLIP)) for v in r["apd90"]]
         for j, v in enumerate(vals):
-            if v is None or v < 0:
+            if v is None:
                 m[i, j] = 0.0
             else:
                 y[i, j] = (v - T_SHIFT) / T_SCALE
-        if points is not None and r["qt"] < 0 and r["act_first"] >= 0:
-            y[i, 0] = (points[i].bcl - T_SHIFT) / T_SCALE
-            m[i, 0] = 1.0
         y[i, N_REG] = 1.0 if r["ead"] else 0.0
     return y, m
 
@@ -124,9 +135,11 @@ def qt_of(results):
     return np.where(q < 0, np.nan, q)
 
 
-def long_qt(qt, threshold):
-    """The sweep's decision: QT above the threshold or not measurable (NaN)."""
-    return ~(np.asarray(qt) <= threshold)
+def long_qt(qt, threshold, ead=None):
+    """The sweep's decision: QT above the threshold, not measurable (NaN),
+    or an EAD."""
+    out = ~(np.asarray(qt) <= threshold)
+    return out if ead is None else out | np.asarray(ead, bool)
 
 
 def decode(out):

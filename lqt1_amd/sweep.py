@@ -12,7 +12,8 @@
 3. Sweep: grid x grid IKs scale x dose at ISO off/on (BCL 1000 ms, 100 cells,
    M fraction 0.35); every point within 10 ms of the QT threshold or with an
    EAD probability in (0.2, 0.8) is re-run by the solver.  One sweep per QT
-   threshold (`--qt-threshold`, or `--qt-percentiles` of the training QT).
+   threshold (`--qt-threshold`, or `--qt-percentiles` of the training QT
+   and the clinical `--qt-thresholds-ms`).
 4. With `--plain`, the whole grid is also run by the solver once, and the
    steered decisions of every threshold are compared with it point by point.
 """
@@ -39,6 +40,9 @@ def parser():
     ap.add_argument("--qt-threshold", type=float, default=None, help="one QT threshold (ms); default: --qt-percentiles")
     ap.add_argument("--qt-percentiles", default="50,75,90,97.5",
                     help="QT thresholds as percentiles of the training QT at BCL 1000 ms, one sweep each")
+    ap.add_argument("--qt-thresholds-ms", default="450",
+                    help="further QT thresholds in ms (450: a clinical long-QT limit; at BCL 1000 ms QT is "
+                         "its own rate-corrected value), one sweep each; '' for none")
     ap.add_argument("--pca-iso", type=float, default=S.PCA_ISO, help="PCa multiplier under ISO")
     ap.add_argument("--plain", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
@@ -89,14 +93,16 @@ def run(a, log=None):
         f"(EAD in {out['heldout_ead_fraction']:.3f})")
 
     # QT thresholds: the given one, else percentiles of the training QT at
-    # BCL 1000 ms (the median puts the threshold where QT values are
-    # densest, so its band holds the most grid points)
-    q = np.array([r["qt"] for r, p in zip(res, pts) if p.bcl == 1000.0 and r["qt"] > 0])
+    # BCL 1000 ms of the strands without an EAD (the median puts the
+    # threshold where QT values are densest, so its band holds the most grid
+    # points)
+    q = np.array([r["qt"] for r, p in zip(res, pts) if p.bcl == 1000.0 and r["qt"] > 0 and not r["ead"]])
     if a.qt_threshold is not None:
         thrs = [(None, a.qt_threshold)]
     else:
-        pcts = [float(x) for x in a.qt_percentiles.split(",")]
+        pcts = [float(x) for x in a.qt_percentiles.split(",") if x.strip()]
         thrs = [(pc, float(np.percentile(q, pc)) if q.size else 450.0) for pc in pcts]
+        thrs += [(None, float(x)) for x in a.qt_thresholds_ms.split(",") if x.strip()]
     g = np.linspace(0.0, 1.0, a.grid)
     I, D, SS = np.meshgrid(g, g, [0.0, 1.0], indexing="ij")
     grid = [S.Point(float(i), int(s), float(d), 1000.0, 100, 0.35) for i, d, s in zip(I.ravel(), D.ravel(), SS.ravel())]
@@ -128,7 +134,7 @@ def run(a, log=None):
             f"re-checked {chk.sum()} ({r['rechecked_ead_band']} by the EAD band) in {sw.t_recheck:.2f} s")
         if plain is not None:
             qt_p, ead_p = plain
-            lq_p = S.long_qt(qt_p, thr)
+            lq_p = S.long_qt(qt_p, thr, ead_p)
             r.update(speedup=out["t_plain_s"] / (sw.t_surrogate + sw.t_recheck),
                      rechecked_identical=bool(np.array_equal(sw.long_qt[chk], lq_p[chk]) and
                                               np.array_equal(sw.ead[chk], ead_p[chk])),

@@ -97,11 +97,13 @@ int64_t or_strand_run(int n, const uint8_t *celltype, const double *coeff,
 typedef struct {
     double act, rep;
     double vrest, vmax, vmin;
-    int flags; /* bit0 entered, bit1 activated, bit2 repolarised, bit3 EAD */
+    int flags; /* bit0 entered, bit1 activated, bit2 repolarised, bit3 EAD,
+                  bit4 entered depolarised (V > -60 mV at the first step) */
 } or_track;
 void or_track_init(or_track *t);
 void or_track_step(or_track *t, double time, double v);
-/* apd = t_rep - act or -1; returns the EAD flags (1 EAD, 2 no repolarisation). */
+/* apd = t_rep - act or -1; returns the EAD flags (1 EAD, 2 no repolarisation:
+ * within the beat, or V above -60 mV at its first step). */
 int or_track_result(const or_track *t, double *act, double *apd);
 
 /* or_strand_run plus the beat measurements of every cell in beat

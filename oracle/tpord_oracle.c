@@ -584,6 +584,8 @@ void or_track_step(or_track *t, double time, double v)
     if (!(t->flags & 1)) {
         t->flags |= 1;
         t->vrest = v;
+        /* still depolarised from the previous beat: it failed to repolarise */
+        if (v > -60.0) t->flags |= 16;
     }
     if (!(t->flags & 2)) {
         if (v >= -30.0) {
@@ -611,7 +613,7 @@ int or_track_result(const or_track *t, double *act, double *apd)
     if (apd) *apd = (t->flags & 4) ? t->rep - t->act : -1.0;
     int e = 0;
     if (t->flags & 8) e |= 1;
-    if ((t->flags & 2) && !(t->flags & 4)) e |= 2;
+    if (((t->flags & 2) && !(t->flags & 4)) || (t->flags & 16)) e |= 2;
     return e;
 }
 

@@ -23,7 +23,7 @@ def test_steered_sweep_matches_plain_sweep(lqt):
 
     iks = np.linspace(0.0, 1.0, 12)
     dose = np.linspace(0.0, 1.0, 12)
-    qt_all = np.array([r["qt"] for r in res])
+    qt_all = np.array([r["qt"] for r in res if not r["ead"]])
     thr = float(np.median(qt_all[qt_all > 0]))
     sw = S.sweep(net, iks, dose, (0, 1), bcl=1000.0, length=100, mfrac=0.35, qt_threshold=thr, band_ms=10.0,
                  nbeats=nb)
@@ -34,11 +34,12 @@ def test_steered_sweep_matches_plain_sweep(lqt):
     ead_p = np.array([bool(r["ead"]) for r in plain])
     chk = sw.rechecked
     # the solver's own decision on every re-checked point
-    assert np.array_equal(sw.long_qt[chk], S.long_qt(S.qt_of(plain), thr)[chk])
+    assert np.array_equal(sw.long_qt[chk], S.long_qt(S.qt_of(plain), thr, ead_p)[chk])
     assert np.array_equal(sw.ead[chk], ead_p[chk])
     # what the network gains: fewer solver runs, and agreement elsewhere
     frac = chk.mean()
-    agree = np.mean(sw.long_qt[~chk] == (qt_p[~chk] > thr)) if (~chk).any() else 1.0
+    lq_p = S.long_qt(S.qt_of(plain), thr, ead_p)
+    agree = np.mean(sw.long_qt[~chk] == lq_p[~chk]) if (~chk).any() else 1.0
     mae = np.nanmean(np.abs(sw.qt_err)) if sw.qt_err.size else 0.0
     print(f"threshold {thr:.1f} ms: re-checked {chk.sum()} of {chk.size} points, agreement off the band "
           f"{agree:.3f}, surrogate |QT err| on re-checked {mae:.2f} ms, plain {t_plain:.2f} s, "
@@ -77,5 +78,41 @@ def test_unmeasurable_qt_counts_as_long(lqt):
     assert np.array_equal(np.isnan(sw.qt), np.isnan(qt_p))
     ok = ~np.isnan(qt_p)
     assert np.array_equal(sw.qt[ok], qt_p[ok])
+    ead_p = np.array([bool(r["ead"]) for r in plain])
+    assert np.array_equal(sw.ead, ead_p)
     assert sw.long_qt[~ok].all()
-    assert np.array_equal(sw.long_qt, S.long_qt(qt_p, thr))
+    assert np.array_equal(sw.long_qt, S.long_qt(qt_p, thr, ead_p))
+
+
+def test_rechecked_ead_positives_are_the_solvers(lqt):
+    """Re-checked points that include early afterdepolarisations (long
+    strands with many M cells under ISO and PCa x 4, found by scanning a few
+    shapes) carry the solver's own EAD and long-QT decisions."""
+    from lqt1_amd import surrogate as S
+
+    iks = np.array([0.0, 0.05, 0.1, 0.2, 0.3, 0.5, 0.7, 1.0])
+    dose = np.array([0.0, 1.0])
+    I, D, SS = np.meshgrid(iks, dose, [1.0], indexing="ij")
+    nb = 10
+    tried = []
+    for length, mfrac, bcl in ((100, 0.35, 1000.0), (120, 0.4, 1000.0), (100, 0.5, 1500.0), (160, 0.3, 800.0),
+                               (80, 0.25, 1000.0), (60, 0.2, 600.0)):
+        grid = [S.Point(float(a), int(s), float(d), bcl, length, mfrac) for a, d, s in zip(I.ravel(), D.ravel(), SS.ravel())]
+        plain, _ = S.simulate(grid, nbeats=nb)
+        ead_p = np.array([bool(r["ead"]) for r in plain])
+        tried.append((length, mfrac, bcl, int(ead_p.sum())))
+        if ead_p.any() and not ead_p.all():
+            break
+    else:
+        pytest.fail(f"no strand shape gives both EAD and EAD-free strands (length, M fraction, BCL, EADs): {tried}")
+    net = S.MLP(seed=4)  # untrained: the probability band below re-checks every point
+    qt_p = S.qt_of(plain)
+    thr = float(np.nanmedian(qt_p[~ead_p])) if (~ead_p).any() else 400.0
+    sw = S.sweep(net, iks, dose, (1,), bcl=bcl, length=length, mfrac=mfrac, qt_threshold=thr, band_ms=10.0,
+                 p_band=(-1.0, 2.0), nbeats=nb)
+    print(f"{length} cells, M fraction {mfrac}, BCL {bcl:.0f}: EAD at {int(ead_p.sum())} of {ead_p.size} points")
+    assert sw.rechecked.all()
+    assert sw.ead.sum() == ead_p.sum() > 0
+    assert np.array_equal(sw.ead, ead_p)
+    assert np.array_equal(sw.long_qt, S.long_qt(qt_p, thr, ead_p))
+    assert sw.long_qt[ead_p].all()

@@ -166,3 +166,44 @@ def test_annulus_fp32_maps_within_gates(lqt):
           f"max |d act| {d_act:.3f} ms, APD90 {p64.min():.1f}-{p64.max():.1f} ms, max |d APD90| {d_apd:.3f} ms")
     assert d_act <= DT + 1e-9
     assert d_apd <= 1.0
+
+
+def test_annulus_fp32_maps_after_s2_within_gates(lqt):
+    """The 2D annulus after its S2 (re-entry): over [S2, 1000 ms) and over
+    [1000, 2000) ms every voxel activates and repolarises in both precisions,
+    FP32 within one step of FP64 in activation and within 1 ms in APD90."""
+    from lqt1_amd import geometry as G
+
+    geo = G.annulus_2d(seed=0)
+    ci = float(geo.info["s2_ci"])
+    windows = ((ci, 1000.0), (1000.0, 2000.0))
+    maps = []
+    for prec in (lqt.FP64, lqt.FP32):
+        T = lqt.Tissue(geo.label, None, geo.dx, 0.154, 0.154, lqt.preset("config1"), prec, DT)
+        T.set_stimulus(geo.stim_sites, None, bcl=1000.0, duration=3.0, strength=-52.0, s2_sites=geo.s2_sites,
+                       s2_time=ci, s2_duration=3.0, s2_strength=-104.0)
+        t, out = 0.0, []
+        for lo, hi in windows:
+            k = 0
+            while t < lo:  # the tissue clock's own accumulator
+                t += DT
+                k += 1
+            T.step(k)
+            T.track(lo, hi)
+            k = 0
+            while t < hi:
+                t += DT
+                k += 1
+            T.step(k)
+            out.append(T.track_results())
+        maps.append(out)
+        T.close()
+    for w, (lo, hi) in enumerate(windows):
+        (a64, p64), (a32, p32) = maps[0][w], maps[1][w]
+        assert np.all(a64 >= 0) and np.all(a32 >= 0), (lo, hi)
+        assert np.all(p64 >= 0) and np.all(p32 >= 0), (lo, hi)
+        d_act, d_apd = np.max(np.abs(a32 - a64)), np.max(np.abs(p32 - p64))
+        print(f"annulus [{lo:.0f}, {hi:.0f}) ms: activation {a64.min():.2f}-{a64.max():.2f} ms, max |d act| "
+              f"{d_act:.3f} ms, APD90 {p64.min():.1f}-{p64.max():.1f} ms, max |d APD90| {d_apd:.3f} ms")
+        assert d_act <= DT + 1e-9
+        assert d_apd <= 1.0

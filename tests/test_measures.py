@@ -46,6 +46,11 @@ def test_tracker_ead_and_failure():
     # never repolarising within the beat
     w = _ap(t, 60.0, 2000.0)
     assert O.track_trace(t, w)[2] & 2
+    # entering the beat still depolarised from the one before (its first V
+    # above -60 mV), even if it then falls to rest within the beat
+    late = np.where(t < 150.0, -20.0 - 0.4 * t, -87.0)
+    assert O.track_trace(t, late)[2] & 2
+    assert O.track_trace(t, np.where(t < 150.0, -61.0, -87.0))[2] == 0
 
 
 def _ecg(t, t_q, qrs_ms, t_peak, t_width, amp=0.06):
