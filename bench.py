@@ -412,6 +412,7 @@ def main():
         return
     if args.profile == "strands":
         r = strands_throughput(A, 512, args.strand_beats, args.seed)
+        r.pop("_specs")
         log(f"strands: {r}")
         return
     if args.profile == "sheet":

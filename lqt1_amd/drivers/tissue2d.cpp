@@ -15,7 +15,10 @@
 //   lqt_tissue2d [--geometry FILE] [--stim FILE] [--dims X Y] [--dx MM]
 //                [--time MS] [--bcl MS] [--preset NAME] [--fp64]
 //                [--dfibre D] [--dcross D] [--outdir DIR] [--vtk-every MS]
-//                [--maps FILE]  (activation time and APD90 per cell: "x y act apd90")
+//                [--maps FILE]  (activation time and APD90 per cell: "x y act apd90", of the first
+//                beat whose upstroke starts in the window; -1 where none)
+//                [--maps-window FROM TO]  (ms of the tissue clock, default 0 to one BCL;
+//                tracking stops at TO)
 //                [--layer Z] [--island X0 X1 Y0 Y1] [--s2-time MS]
 //                [--labels-out FILE]
 #include <stdio.h>
@@ -46,6 +49,7 @@ int main(int argc, char** argv) {
     int X = 690, Y = 489, layer = 157;       // new_11.cu:29-30, :1734
     double dx = 0.2, dt = 0.02;              // new_11.cu:62-63
     double tmax = 30000.0, bcl = 1000.0, vtk_every = 10.0;  // :1751-1755
+    double maps_from = 0.0, maps_to = -1.0;  // the --maps window; TO < 0: one BCL
     double dfib = 0.154, dcross = 0.0385;    // D1, D2 of new_11.cu:49-51
     int isl[4] = {550, 689, 150, 487};       // new_11.cu:1484
     const int s2box[6] = {300, 400, 200, 350, 11, 13};  // new_11.cu:1470
@@ -74,6 +78,10 @@ int main(int argc, char** argv) {
         } else if (a == "--s2-time") s2_time = atof(next());
         else if (a == "--labels-out") labels_out = next();
         else if (a == "--maps") maps = next();
+        else if (a == "--maps-window") {
+            maps_from = atof(next());
+            maps_to = atof(next());
+        }
         else {
             fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
@@ -166,7 +174,9 @@ int main(int argc, char** argv) {
                                 s2cells.data()))
         return die("lqt_tissue_set_stimulus");
 
-    if (!maps.empty() && lqt_tissue_track(t, 0.0, 1e300)) return die("lqt_tissue_track");
+    // the trackers hold one beat: track from maps_from to maps_to (one BCL by default)
+    if (maps_to < 0.0) maps_to = maps_from + bcl;
+    if (!maps.empty() && lqt_tissue_track(t, maps_from, maps_to)) return die("lqt_tissue_track");
 
     const int64_t per_vtk = (int64_t)(vtk_every / dt);
     const int64_t per_ms = (int64_t)(1 / dt);

@@ -20,7 +20,10 @@
 //   lqt_tissue3d --geometry FILE --dims X Y Z [--stim FILE] [--dx MM]
 //                [--time MS] [--bcl MS] [--preset NAME] [--fp64]
 //                [--dfibre D] [--dcross D] [--outdir DIR] [--vtk-every MS]
-//                [--maps FILE]  (activation time and APD90 per voxel: "x y z act apd90")
+//                [--maps FILE]  (activation time and APD90 per voxel: "x y z act apd90", of the first
+//                beat whose upstroke starts in the window; -1 where none)
+//                [--maps-window FROM TO]  (ms of the tissue clock, default 0 to one BCL;
+//                tracking stops at TO)
 //                [--vtk-stride N] [--island X Y Z R] [--s2-box X0 X1 Y0 Y1 Z0 Z1]
 //                [--s2-time MS] [--window MS] [--labels-out FILE]
 #include <math.h>
@@ -52,7 +55,7 @@ int main(int argc, char** argv) {
     double dfib = 0.154, dcross = 0.154;  // GPU3D.cu:37-38 (isotropic)
     double island[4] = {597, 336, 336, 75};         // 199*3, 112*3, 112*3, 25*3 (GPU3D.cu:1454-1456)
     int s2box[6] = {300, 687, 120, 486, 420, 534};  // 100*3 .. 229*3 etc. (GPU3D.cu:1438)
-    double s2_time = -1.0, window = 82.0;
+    double s2_time = -1.0, window = 82.0, maps_from = 0.0, maps_to = -1.0;
     int vtk_stride = 3;
     bool fp64 = false;
     for (int i = 1; i < argc; ++i) {
@@ -82,6 +85,10 @@ int main(int argc, char** argv) {
         else if (a == "--window") window = atof(next());
         else if (a == "--labels-out") labels_out = next();
         else if (a == "--maps") maps = next();
+        else if (a == "--maps-window") {
+            maps_from = atof(next());
+            maps_to = atof(next());
+        }
         else {
             fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
@@ -187,7 +194,9 @@ int main(int argc, char** argv) {
                                 s2cells.data()))
         return die("lqt_tissue_set_stimulus");
 
-    if (!maps.empty() && lqt_tissue_track(t, 0.0, 1e300)) return die("lqt_tissue_track");
+    // the trackers hold one beat: track from maps_from to maps_to (one BCL by default)
+    if (maps_to < 0.0) maps_to = maps_from + bcl;
+    if (!maps.empty() && lqt_tissue_track(t, maps_from, maps_to)) return die("lqt_tissue_track");
 
     const int64_t per_vtk = (int64_t)(vtk_every / dt + 0.5);
     const int64_t per_ms = (int64_t)(1.0 / dt + 0.5);

@@ -13,7 +13,9 @@ start at 81 ms (cut to 1 ms by the 82 ms window of GPU3D.cu:1757) and at
   stimulus rule, to the 6 digits the file keeps; at 83 ms the same oracle
   without the 82 ms window differs by about 11 mV, so the comparison sees
   the window;
-* the activation map the program writes with --maps against the oracle's.
+* the activation map the program writes with --maps against the oracle's;
+* over a whole beat, both columns of the --maps file (activation, APD90)
+  against the library's trackers, and the --maps-window cut-off.
 """
 import os
 import subprocess
@@ -181,3 +183,64 @@ def test_ventricle_program_matches_oracle(lqt, oracle, tmp_path):
     d_win = np.max(np.abs(nowin[4150] - ref[4150]))
     print(f"the 82 ms window changes V at 83 ms by up to {d_win:.1f} mV")
     assert d_win > 5.0
+
+
+def _driver_steps(tmax, dt=0.02):
+    """Steps the program's time loop takes to pass tmax (time <= tmax, time += dt)."""
+    tt, n = 0.0, 0
+    while tt <= tmax:
+        tt += dt
+        n += 1
+    return n
+
+
+def test_ventricle_program_maps_over_a_beat(lqt, tmp_path):
+    """--maps over a whole beat (BCL 500 ms, 480 ms run): both columns of the
+    file equal the library's trackers on the same tissue (the trackers are
+    pinned to the oracle in the tissue-track tests); a --maps-window that ends
+    at the median activation time stops tracking there, so no voxel gets an
+    APD90 and only the upstrokes before it an activation time."""
+    tissue, lab, f, rim, tact = _files(tmp_path)
+    exe = os.path.join(ROOT, "lqt1_amd", "bin", "lqt_tissue3d")
+    common = [exe, "--geometry", str(tmp_path / "geo.txt"), "--dims", str(X), str(Y), str(Z),
+              "--stim", str(tmp_path / "stim.cnd"), "--fp64", "--dfibre", "0.154", "--dcross", "0.0385",
+              "--vtk-every", "0", "--island", *[str(v) for v in ISLAND], "--bcl", "500"]
+    r = subprocess.run(common + ["--time", "480", "--labels-out", str(tmp_path / "lab.txt"),
+                                 "--maps", str(tmp_path / "beat.txt")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    # the same run through the library: cell types as the program assigned them,
+    # the fibre as the file holds it in single precision, the profile's sites
+    rl = np.loadtxt(tmp_path / "lab.txt", dtype=int)
+    ct = np.full((X, Y, Z), lqt.NONE, np.uint8)
+    ct[rl[:, 0], rl[:, 1], rl[:, 2]] = rl[:, 4]
+    geo = np.loadtxt(tmp_path / "geo.txt")
+    gi = geo[:, :3].astype(int)
+    fib = np.zeros((X, Y, Z, 3), np.float32)
+    fib[gi[:, 0], gi[:, 1], gi[:, 2]] = geo[:, 3:6].astype(np.float32)
+    sites = np.ravel_multi_index(rim.T, (X, Y, Z))
+    offs = np.float32(1000.0) * tact.astype(np.float32)
+    T = lqt.Tissue(ct, fib, 0.2, 0.154, 0.0385, lqt.preset("gpu3d"), lqt.FP64, 0.02)
+    T.set_stimulus(sites, offs, bcl=500.0, duration=3.0, strength=-52.0, window=82.0)
+    T.track(0.0, 500.0)
+    T.step(_driver_steps(480.0))
+    act, apd = T.track_results()
+    T.close()
+    mp = np.loadtxt(tmp_path / "beat.txt")
+    assert mp.shape == (tissue.sum(), 5)
+    assert np.array_equal(mp[:, :3].astype(int), np.argwhere(tissue))
+    assert np.all(act > 0) and np.all(apd > 100.0), "every voxel activates and repolarises within the beat"
+    assert np.max(np.abs(mp[:, 3] - act)) < 0.006
+    assert np.max(np.abs(mp[:, 4] - apd)) < 0.006
+    print(f"maps over one beat: {act.size} voxels, activation {act.min():.2f}-{act.max():.2f} ms, "
+          f"APD90 {apd.min():.2f}-{apd.max():.2f} ms, equal to the library's trackers")
+    # a window that ends at the median activation time (between two step starts)
+    w = float(np.median(act)) + 0.01
+    r = subprocess.run(common + ["--time", f"{w + 10.0:.2f}", "--maps", str(tmp_path / "early.txt"),
+                                 "--maps-window", "0", f"{w:.2f}"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    early = np.loadtxt(tmp_path / "early.txt")
+    assert np.all(early[:, 4] == -1.0)
+    hit = act < w
+    assert 0 < hit.sum() < act.size
+    assert np.max(np.abs(early[hit, 3] - act[hit])) < 0.006
+    assert np.all(early[~hit, 3] == -1.0)
