@@ -224,9 +224,14 @@ struct GroupA {
 
 // Group B: ICaL/ICaNa/ICaK, IKr, IK1, INaCa, IpCa, the background currents
 // and Jrel, with the gates d ff fs fcaf fcas jca nca ffp fcafp xrf xrs xk1
-// Jrelnp Jrelp.  The two groups take about the same issue time: 439 VALU
-// instructions, 93 of them transcendental, against 507 with 76 (a
-// transcendental instruction occupies the SIMD four times as long).
+// Jrelnp Jrelp.  The split strand kernel runs the groups on different
+// wavefronts that meet at a barrier every step; this grouping is the one
+// with the shortest measured step (1.79 us for a 100-cell strand on gfx950),
+// although group B issues more (about 2,340 cycles per step before the
+// barrier against 2,110 for group A, at 4 cycles per wave64 VALU instruction
+// and 8 per transcendental one): with IK1 in group A the two issue about
+// 2,230 each and the step takes 1.86 us, one wavefront per SIMD being bound
+// by its dependency chains rather than by issue.
 template <class R>
 struct GroupB {
     R ICaL, ICaNa, ICaK, IKr, IK1, INaCa_i, INaCa_ss, IpCa, ICab, Jrel, INab, IKb;
