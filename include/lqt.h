@@ -196,7 +196,12 @@ lqt_status lqt_tissue_step_async(lqt_tissue *t, int64_t nsteps, void *stream);
 lqt_status lqt_tissue_time(const lqt_tissue *t, double *time_ms, int64_t *step);
 /* V of every tissue voxel in compacted (x,y,z) order, out[nvox]. */
 lqt_status lqt_tissue_read_v(lqt_tissue *t, double *out);
-/* All 40 states of tissue voxel k in the 40-line order, out[40]. */
+/* All 40 states of tissue voxel k in the 40-line order, out[40].  An FP32
+ * tissue holds V, the gates and the fast concentrations in float and the five
+ * slow concentrations (nai, ki, cansr, cajsr, CaMKt) as float offsets from
+ * reference values (the initial values at creation, or those a checkpoint
+ * carries after lqt_tissue_load), so a value written here comes back rounded
+ * to that precision. */
 lqt_status lqt_tissue_read_state(lqt_tissue *t, int64_t k, double *out);
 lqt_status lqt_tissue_write_state(lqt_tissue *t, int64_t k, const double *in);
 /* Counts of V < -90 mV, V > 200 mV and non-finite V (GPU3D.cu:1796-1829),
@@ -218,8 +223,10 @@ lqt_status lqt_tissue_track_results(lqt_tissue *t, double *act, double *apd90);
 /* Legacy VTK STRUCTURED_POINTS of V on every `stride`-th voxel, -100 outside
  * (GPU3D.cu:123-163 writes stride 3). */
 lqt_status lqt_tissue_write_vtk(lqt_tissue *t, const char *path, int stride);
-/* Binary checkpoint of time and full state; restores into a tissue of the
- * same geometry. */
+/* Binary checkpoint of time and full state as stored (with the FP32 slow
+ * concentrations' reference values); restores into a tissue of the same
+ * geometry and precision, also from checkpoints of the earlier format with
+ * those concentrations in double. */
 lqt_status lqt_tissue_save(lqt_tissue *t, const char *path);
 lqt_status lqt_tissue_load(lqt_tissue *t, const char *path);
 /* Launch shapes of the fused step kernel (workgroup size; whether

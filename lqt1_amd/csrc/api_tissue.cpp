@@ -31,7 +31,9 @@ struct lqt_tissue {
     void* grid[2] = {nullptr, nullptr};
     int cur = 0;
     void* r = nullptr;
-    double* d = nullptr;
+    // slow accumulators: double (FP64), or float offsets from dref (FP32)
+    void* d = nullptr;
+    double dref[ND] = {0, 0, 0, 0, 0};
     void* fd = nullptr;  // fibre and drift, chunk-major (6 slots)
     int32_t* gdst = nullptr;   // ghost cells (padded index) and their sources,
     int32_t* gsrc = nullptr;   // for refreshes outside the step kernel
@@ -60,6 +62,16 @@ struct lqt_tissue {
 };
 
 static size_t rsz(const lqt_tissue* t) { return t->fp32 ? sizeof(float) : sizeof(double); }
+static size_t dsz(const lqt_tissue* t) { return t->fp32 ? sizeof(float) : sizeof(double); }
+
+// Stored form of slow slot j of value x, and back (FP32: offset from dref).
+static void slow_put(const lqt_tissue* t, int j, double x, void* out) {
+    if (t->fp32) *(float*)out = (float)(x - t->dref[j]);
+    else *(double*)out = x;
+}
+static double slow_get(const lqt_tissue* t, int j, const void* in) {
+    return t->fp32 ? t->dref[j] + (double)*(const float*)in : *(const double*)in;
+}
 
 
 // Ghost map of GPU3D.cu:1487-1604: every non-tissue cell with a tissue cell
@@ -244,7 +256,8 @@ static void fill_args(lqt_tissue* t, TissueArgs<R>& a) {
     a.inv_4h2 = (R)(1.0 / (4.0 * t->dx * t->dx));
     a.inv_2h = (R)(1.0 / (2.0 * t->dx));
     a.sr = (R*)t->r;
-    a.sd = t->d;
+    a.sd = (typename SlowStore<R>::type*)t->d;
+    for (int j = 0; j < ND; ++j) a.sref.x[j] = t->dref[j];
     a.consts = (const Consts<R>*)t->consts;
     a.dt = (R)t->dt;
     a.stim_on = 0;
@@ -386,7 +399,7 @@ lqt_status lqt_tissue_create(const lqt_geometry* geo, const lqt_params* params, 
         LQT_HIP(hipMalloc(&t->grid[0], rs * npad));
         LQT_HIP(hipMalloc(&t->grid[1], rs * npad));
         LQT_HIP(hipMalloc(&t->r, rs * NR * t->pitch));
-        LQT_HIP(hipMalloc((void**)&t->d, sizeof(double) * ND * t->pitch));
+        LQT_HIP(hipMalloc(&t->d, dsz(t) * ND * t->pitch));
         LQT_HIP(hipMalloc((void**)&t->guard, sizeof(unsigned long long) * 3));
         LQT_HIP(hipMemset(t->guard, 0, sizeof(unsigned long long) * 3));
         if (t->aniso) {
@@ -660,7 +673,9 @@ lqt_status lqt_tissue_read_state(lqt_tissue* t, int64_t k, double* out) {
             LQT_HIP(hipMemcpy(buf, (unsigned char*)t->r + rs * chunk_slot(k, m.slot, NR), rs, hipMemcpyDeviceToHost));
             val = t->fp32 ? (double)*(float*)buf : *(double*)buf;
         } else if (m.kind == 2) {
-            LQT_HIP(hipMemcpy(&val, t->d + chunk_slot(k, m.slot, ND), sizeof(double), hipMemcpyDeviceToHost));
+            LQT_HIP(hipMemcpy(buf, (unsigned char*)t->d + dsz(t) * chunk_slot(k, m.slot, ND), dsz(t),
+                              hipMemcpyDeviceToHost));
+            val = slow_get(t, m.slot, buf);
         }
         out[j] = val;
     }
@@ -681,7 +696,9 @@ lqt_status lqt_tissue_write_state(lqt_tissue* t, int64_t k, const double* in) {
         } else if (m.kind == 1) {
             LQT_HIP(hipMemcpy((unsigned char*)t->r + rs * chunk_slot(k, m.slot, NR), buf, rs, hipMemcpyHostToDevice));
         } else if (m.kind == 2) {
-            LQT_HIP(hipMemcpy(t->d + chunk_slot(k, m.slot, ND), &in[j], sizeof(double), hipMemcpyHostToDevice));
+            slow_put(t, m.slot, in[j], buf);
+            LQT_HIP(hipMemcpy((unsigned char*)t->d + dsz(t) * chunk_slot(k, m.slot, ND), buf, dsz(t),
+                              hipMemcpyHostToDevice));
         }
     }
     // refresh the ghost mirrors of the current grid
@@ -734,7 +751,7 @@ lqt_status lqt_tissue_autotune(lqt_tissue* t, int trial_steps, int* best, double
     LQT_CHECK(t && trial_steps > 0, LQT_ERR_ARG, "lqt_tissue_autotune: bad argument");
     LQT_HIP(hipStreamSynchronize(t->stream));
     const size_t rs = rsz(t);
-    const size_t nr = rs * NR * t->pitch, nd = sizeof(double) * ND * t->pitch, ng = rs * t->ncell_pad;
+    const size_t nr = rs * NR * t->pitch, nd = dsz(t) * ND * t->pitch, ng = rs * t->ncell_pad;
     void *sr = nullptr, *sd = nullptr, *sg0 = nullptr, *sg1 = nullptr, *sgu = nullptr;
     auto release = [&]() {
         hipFree(sr);
@@ -858,9 +875,9 @@ lqt_status lqt_tissue_last_kernel_us(const lqt_tissue* t, double* us_step_kernel
 lqt_status lqt_tissue_bytes_per_step(const lqt_tissue* t, double* bytes) {
     LQT_CHECK(t && bytes, LQT_ERR_ARG, "lqt_tissue_bytes_per_step: bad argument");
     const double rs = (double)rsz(t);
-    // per voxel: V read + write, NR R-slots and ND doubles read + written,
+    // per voxel: V read + write, NR R-slots and ND slow slots read + written,
     // class byte and grid index, fibre + drift (anisotropic)
-    double per = 2.0 * (rs + NR * rs + ND * 8.0) + 1.0 + 4.0;
+    double per = 2.0 * (rs + NR * rs + ND * (double)dsz(t)) + 1.0 + 4.0;
     if (t->aniso) per += (t->nz == 1 ? 4.0 : 6.0) * rs;  // fibre + drift (x, y only in 2D)
     // plus the ghost refresh: destination index read and V written per
     // ghost cell, one list offset per wavefront
@@ -873,8 +890,24 @@ lqt_status lqt_tissue_bytes_per_step(const lqt_tissue* t, double* bytes) {
 static lqt_status tissue_upload_state(lqt_tissue* t, const std::vector<double>& s40) {
     const size_t rs = rsz(t);
     std::vector<unsigned char> r(rs * NR * t->pitch, 0);
-    std::vector<double> d(ND * t->pitch, 0.0);
+    std::vector<unsigned char> d(dsz(t) * ND * t->pitch, 0);
     std::vector<unsigned char> g(rs * t->ncell_pad);
+    // FP32: the offsets' reference values, midway between each slow slot's
+    // smallest and largest value over the voxels (the initial values, when
+    // every voxel starts from the same state)
+    for (int j = 0; j < ND; ++j) t->dref[j] = 0.0;
+    if (t->fp32 && t->nvox > 0) {
+        for (int j = 0; j < 40; ++j) {
+            const Ref40 m = kRef40[j];
+            if (m.kind != 2) continue;
+            double lo = s40[j], hi = s40[j];
+            for (int64_t k = 1; k < t->nvox; ++k) {
+                lo = std::min(lo, s40[(size_t)k * 40 + j]);
+                hi = std::max(hi, s40[(size_t)k * 40 + j]);
+            }
+            t->dref[m.slot] = 0.5 * (lo + hi);
+        }
+    }
     for (int64_t i = 0; i < t->ncell_pad; ++i) {
         if (t->fp32) ((float*)g.data())[i] = -90.0f;
         else ((double*)g.data())[i] = -90.0;
@@ -891,12 +924,12 @@ static lqt_status tissue_upload_state(lqt_tissue* t, const std::vector<double>& 
                 if (t->fp32) ((float*)r.data())[idx] = (float)val;
                 else ((double*)r.data())[idx] = val;
             } else if (m.kind == 2) {
-                d[(size_t)chunk_slot(k, m.slot, ND)] = val;
+                slow_put(t, m.slot, val, d.data() + dsz(t) * (size_t)chunk_slot(k, m.slot, ND));
             }
         }
     }
     LQT_HIP(hipMemcpy(t->r, r.data(), r.size(), hipMemcpyHostToDevice));
-    LQT_HIP(hipMemcpy(t->d, d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice));
+    LQT_HIP(hipMemcpy(t->d, d.data(), d.size(), hipMemcpyHostToDevice));
     LQT_HIP(hipMemcpy(t->grid[t->cur], g.data(), g.size(), hipMemcpyHostToDevice));
     LQT_HIP(hipMemcpy(t->grid[t->cur ^ 1], g.data(), g.size(), hipMemcpyHostToDevice));
     launch_ghost_fill(t->fp32, t->grid[t->cur], t->gdst, t->gsrc, t->nghost, t->stream);
@@ -1005,25 +1038,28 @@ lqt_status lqt_tissue_write_vtk(lqt_tissue* t, const char* path, int stride) {
     return LQT_OK;
 }
 
-// Checkpoint: magic, header (dims, voxels, precision, step), time, then the
-// state arrays as stored (chunk-major) and the current V grid.
-static const char kMagic[8] = {'L', 'Q', 'T', 'T', 'I', 'S', '2', 0};
+// Checkpoint: magic, header (dims, voxels, precision, step), time, the slow
+// slots' reference values, then the state arrays as stored (chunk-major)
+// and the current V grid.  Checkpoints of the earlier format (slow slots
+// stored as double in both precisions, no reference values) are read too.
+static const char kMagic[8] = {'L', 'Q', 'T', 'T', 'I', 'S', '3', 0};
+static const char kMagic2[8] = {'L', 'Q', 'T', 'T', 'I', 'S', '2', 0};
 
 lqt_status lqt_tissue_save(lqt_tissue* t, const char* path) {
     LQT_CHECK(t && path, LQT_ERR_ARG, "lqt_tissue_save: bad argument");
     LQT_HIP(hipStreamSynchronize(t->stream));
     const size_t rs = rsz(t);
-    std::vector<unsigned char> r(rs * NR * t->pitch), g(rs * t->ncell_pad);
-    std::vector<double> d(ND * t->pitch);
+    std::vector<unsigned char> r(rs * NR * t->pitch), d(dsz(t) * ND * t->pitch), g(rs * t->ncell_pad);
     LQT_HIP(hipMemcpy(r.data(), t->r, r.size(), hipMemcpyDeviceToHost));
-    LQT_HIP(hipMemcpy(d.data(), t->d, sizeof(double) * d.size(), hipMemcpyDeviceToHost));
+    LQT_HIP(hipMemcpy(d.data(), t->d, d.size(), hipMemcpyDeviceToHost));
     LQT_HIP(hipMemcpy(g.data(), t->grid[t->cur], g.size(), hipMemcpyDeviceToHost));
     FILE* f = fopen(path, "wb");
     LQT_CHECK(f, LQT_ERR_IO, "cannot open %s", path);
     const int64_t hdr[6] = {t->nx, t->ny, t->nz, t->nvox, (int64_t)t->fp32, t->step};
     bool ok = fwrite(kMagic, 1, 8, f) == 8 && fwrite(hdr, sizeof(hdr), 1, f) == 1 &&
-              fwrite(&t->time, sizeof(double), 1, f) == 1 && fwrite(r.data(), 1, r.size(), f) == r.size() &&
-              fwrite(d.data(), sizeof(double), d.size(), f) == d.size() && fwrite(g.data(), 1, g.size(), f) == g.size();
+              fwrite(&t->time, sizeof(double), 1, f) == 1 && fwrite(t->dref, sizeof(double), ND, f) == (size_t)ND &&
+              fwrite(r.data(), 1, r.size(), f) == r.size() && fwrite(d.data(), 1, d.size(), f) == d.size() &&
+              fwrite(g.data(), 1, g.size(), f) == g.size();
     fclose(f);
     LQT_CHECK(ok, LQT_ERR_IO, "short write to %s", path);
     return LQT_OK;
@@ -1036,24 +1072,46 @@ lqt_status lqt_tissue_load(lqt_tissue* t, const char* path) {
     char magic[8];
     int64_t hdr[6];
     double time;
-    bool ok = fread(magic, 1, 8, f) == 8 && !memcmp(magic, kMagic, 8) && fread(hdr, sizeof(hdr), 1, f) == 1 &&
-              fread(&time, sizeof(double), 1, f) == 1;
+    bool ok = fread(magic, 1, 8, f) == 8 && (!memcmp(magic, kMagic, 8) || !memcmp(magic, kMagic2, 8)) &&
+              fread(hdr, sizeof(hdr), 1, f) == 1 && fread(&time, sizeof(double), 1, f) == 1;
     if (!ok || hdr[0] != t->nx || hdr[1] != t->ny || hdr[2] != t->nz || hdr[3] != t->nvox || hdr[4] != (int64_t)t->fp32) {
         fclose(f);
         return set_error(LQT_ERR_IO, "%s: not a checkpoint of this geometry and precision", path);
     }
-    const size_t rs = rsz(t);
-    std::vector<unsigned char> r(rs * NR * t->pitch), g(rs * t->ncell_pad);
-    std::vector<double> d(ND * t->pitch);
-    ok = fread(r.data(), 1, r.size(), f) == r.size() && fread(d.data(), sizeof(double), d.size(), f) == d.size() &&
-         fread(g.data(), 1, g.size(), f) == g.size();
+    const bool v2 = !memcmp(magic, kMagic2, 8);
+    double dref[ND] = {0, 0, 0, 0, 0};
+    const size_t rs = rsz(t), ds = v2 ? sizeof(double) : dsz(t);
+    std::vector<unsigned char> r(rs * NR * t->pitch), d(ds * ND * t->pitch), g(rs * t->ncell_pad);
+    ok = (v2 || fread(dref, sizeof(double), ND, f) == (size_t)ND) && fread(r.data(), 1, r.size(), f) == r.size() &&
+         fread(d.data(), 1, d.size(), f) == d.size() && fread(g.data(), 1, g.size(), f) == g.size();
     fclose(f);
     LQT_CHECK(ok, LQT_ERR_IO, "%s: truncated checkpoint", path);
+    if (v2 && t->fp32) {
+        // slow slots stored as double: re-express them as offsets
+        const double* dd = (const double*)d.data();
+        for (int j = 0; j < ND; ++j) {
+            double lo = 0, hi = 0;
+            for (int64_t k = 0; k < t->nvox; ++k) {
+                const double x = dd[chunk_slot(k, j, ND)];
+                lo = k ? std::min(lo, x) : x;
+                hi = k ? std::max(hi, x) : x;
+            }
+            dref[j] = 0.5 * (lo + hi);
+        }
+        std::vector<unsigned char> df(sizeof(float) * ND * t->pitch, 0);
+        for (int64_t k = 0; k < t->nvox; ++k)
+            for (int j = 0; j < ND; ++j) {
+                const int64_t i = chunk_slot(k, j, ND);
+                ((float*)df.data())[i] = (float)(dd[i] - dref[j]);
+            }
+        d.swap(df);
+    }
     LQT_HIP(hipStreamSynchronize(t->stream));
     LQT_HIP(hipMemcpy(t->r, r.data(), r.size(), hipMemcpyHostToDevice));
-    LQT_HIP(hipMemcpy(t->d, d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice));
+    LQT_HIP(hipMemcpy(t->d, d.data(), d.size(), hipMemcpyHostToDevice));
     LQT_HIP(hipMemcpy(t->grid[t->cur], g.data(), g.size(), hipMemcpyHostToDevice));
     LQT_HIP(hipMemcpy(t->grid[t->cur ^ 1], g.data(), g.size(), hipMemcpyHostToDevice));
+    for (int j = 0; j < ND; ++j) t->dref[j] = dref[j];
     t->time = time;
     t->step = hdr[5];
     return LQT_OK;

@@ -52,7 +52,9 @@ struct TissueArgs {
     R dcross, ddiff;           // D_cross, D_fibre - D_cross
     R inv_h2, inv_4h2, inv_2h; // 1/dx^2, 1/(4dx^2), 1/(2dx)
     R* sr;                     // state, chunk-major: NR slots of R per chunk
-    double* sd;                // state, chunk-major: ND slots of double per chunk
+    typename SlowStore<R>::type* sd;  // state, chunk-major: ND slow slots per chunk (double, or
+                                     // float offsets from sref in FP32)
+    SlowRef sref;
     const Consts<R>* consts;
     R dt;
     // stimulus of this step

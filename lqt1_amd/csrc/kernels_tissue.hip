@@ -184,7 +184,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W))) void tis
     const R ist = stimulus(A, k);
 
     CellState<R> s;
-    load_state_chunked(s, A.sr, A.sd, k);
+    load_state_chunked(s, A.sr, A.sd, k, A.sref);
     // cls: bits 0-1 class, bits 2-6 ghost cells mirroring this voxel,
     // bit 7 V held at -100 mV (new_11.cu:1247-1248)
     const int c = A.cls[k];
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W))) void tis
     const Consts<R> K = *A.consts;  // uniform: scalar loads
     R vn = tpord_update<R, false>(s, vc, c & 3, K, A.dt, ist, du, nullptr);
     if (held) vn = R(-100);
-    store_state_chunked(s, A.sr, A.sd, k);
+    store_state_chunked(s, A.sr, A.sd, k, A.sref);
     *(R*)((char*)A.vnew + pb) = vn;
     // lanes past the last voxel have returned (they are the top lanes of the
     // last wavefront), so every lane here holds a voxel

@@ -120,28 +120,65 @@ __host__ __device__ inline int64_t chunk_slot(int64_t k, int j, int nslot) {
     return ((k >> 6) * nslot + j) * kChunk + (k & 63);
 }
 
+// The tissue's stored slow accumulators (nai, ki, cansr, cajsr, CaMKt):
+// double in the FP64 path; in the FP32 path float offsets from reference
+// values of the tissue (SlowRef, set when the state is written from the
+// host).  The update runs in double in registers either way; storing the
+// offset keeps a step's change, which for ki (about 1e-6 mM per step at
+// 145 mM) is far below one FP32 ULP of the value itself (1.5e-5 mM) but
+// many ULPs of an offset that stays within a few mM of zero.
 template <class R>
-__device__ __forceinline__ void load_state_chunked(CellState<R>& s, const R* r, const double* d, uint32_t k) {
+struct SlowStore {
+    using type = double;
+};
+template <>
+struct SlowStore<float> {
+    using type = float;
+};
+struct SlowRef {
+    double x[ND];
+};
+
+template <class R>
+__device__ __forceinline__ void load_state_chunked(CellState<R>& s, const R* r, const typename SlowStore<R>::type* d,
+                                                   uint32_t k, const SlowRef& ref) {
+    using S = typename SlowStore<R>::type;
     const R* rb = r + (size_t)(k >> 6) * (NR * kChunk) + (k & 63);
-    const double* db = d + (size_t)(k >> 6) * (ND * kChunk) + (k & 63);
+    const S* db = d + (size_t)(k >> 6) * (ND * kChunk) + (k & 63);
 #define LQT_LD(n) s.n = ld_nt(rb, (uint32_t)(RS_##n * kChunk * sizeof(R)));
     LQT_RFIELDS(LQT_LD)
 #undef LQT_LD
-#define LQT_LD(n) s.n = ld_nt(db, (uint32_t)(DS_##n * kChunk * 8));
-    LQT_DFIELDS(LQT_LD)
+    if constexpr (sizeof(S) == 4) {
+#define LQT_LD(n) s.n = ref.x[DS_##n] + (double)ld_nt(db, (uint32_t)(DS_##n * kChunk * sizeof(S)));
+        LQT_DFIELDS(LQT_LD)
 #undef LQT_LD
+    } else {
+        (void)ref;
+#define LQT_LD(n) s.n = ld_nt(db, (uint32_t)(DS_##n * kChunk * sizeof(S)));
+        LQT_DFIELDS(LQT_LD)
+#undef LQT_LD
+    }
 }
 
 template <class R>
-__device__ __forceinline__ void store_state_chunked(const CellState<R>& s, R* r, double* d, uint32_t k) {
+__device__ __forceinline__ void store_state_chunked(const CellState<R>& s, R* r, typename SlowStore<R>::type* d,
+                                                    uint32_t k, const SlowRef& ref) {
+    using S = typename SlowStore<R>::type;
     R* rb = r + (size_t)(k >> 6) * (NR * kChunk) + (k & 63);
-    double* db = d + (size_t)(k >> 6) * (ND * kChunk) + (k & 63);
+    S* db = d + (size_t)(k >> 6) * (ND * kChunk) + (k & 63);
 #define LQT_ST(n) st_nt(s.n, rb, (uint32_t)(RS_##n * kChunk * sizeof(R)));
     LQT_RFIELDS(LQT_ST)
 #undef LQT_ST
-#define LQT_ST(n) st_nt(s.n, db, (uint32_t)(DS_##n * kChunk * 8));
-    LQT_DFIELDS(LQT_ST)
+    if constexpr (sizeof(S) == 4) {
+#define LQT_ST(n) st_nt((float)(s.n - ref.x[DS_##n]), db, (uint32_t)(DS_##n * kChunk * sizeof(S)));
+        LQT_DFIELDS(LQT_ST)
 #undef LQT_ST
+    } else {
+        (void)ref;
+#define LQT_ST(n) st_nt(s.n, db, (uint32_t)(DS_##n * kChunk * sizeof(S)));
+        LQT_DFIELDS(LQT_ST)
+#undef LQT_ST
+    }
 }
 
 // Location of each of the reference's 40 states (TPORd.cc:200-252 order)

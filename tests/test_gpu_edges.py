@@ -2,8 +2,11 @@
 
 * A tissue of one voxel (every stencil neighbour is a ghost mirroring it, so
   the diffusion term is exactly zero) steps exactly as one cell of the cell
-  engine with the same stimulus: identical bits of all 40 states, FP64 and
-  FP32, isotropic and anisotropic.
+  engine with the same stimulus: identical bits of all 40 states in FP64,
+  isotropic and anisotropic.  In FP32 the tissue stores the five slow
+  concentrations as FP32 offsets from reference values after every step
+  (the cell engine keeps them in double), so there every state agrees to
+  1e-5 relative after 1500 steps and V to 1e-3 mV.
 * Strands at the wavefront edges (2, 63, 64, 65 and 128 cells) against the
   FP64 oracle of OneD.cc, in the per-size and the packed layouts.
 * Empty inputs, strands outside the documented 2..512 cells and a
@@ -49,7 +52,14 @@ def test_one_voxel_tissue_is_one_cell(lqt, prec, aniso):
     c.step(k, istim=-52.0)
     c.step(nsteps - k)
     a, b = T.state(0), c.state()[0]
-    assert a[0] > -90.0 and np.max(np.abs(a - b)) == 0.0, np.flatnonzero(a != b)
+    assert a[0] > -90.0
+    if prec == "fp64":
+        assert np.max(np.abs(a - b)) == 0.0, np.flatnonzero(a != b)
+    else:
+        rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-3)
+        print(f"one-voxel FP32 tissue vs cell: max rel {rel.max():.2e}, |dV| {abs(a[0] - b[0]):.2e} mV")
+        assert rel.max() < 1e-5, np.flatnonzero(rel >= 1e-5)
+        assert abs(a[0] - b[0]) < 1e-3
 
 
 def _oracle(O, spec):

@@ -189,6 +189,50 @@ def test_checkpoint_and_vtk(lqt, tmp_path):
     assert T.v_guard()[2] == 0
 
 
+def test_checkpoint_of_the_earlier_format(lqt, tmp_path):
+    """A checkpoint whose FP32 slow concentrations are stored in double (the
+    earlier format, magic LQTTIS2, no reference values) restores the same
+    state: built here from a current checkpoint (LQTTIS3: header, time, 5
+    reference values, the float slots, the 5 slow slots as float offsets,
+    the V grid), it loads into a fresh tissue whose 40 states equal the
+    saved ones to FP32 rounding of the offsets, and both continue alike."""
+    lab, fib, sites = _blob(2, (12, 12, 10))
+    def make():
+        T = lqt.Tissue(lab, fib, 0.2, 0.154, 0.0385, lqt.preset("cpu_default"), lqt.FP32, 0.02)
+        T.set_stimulus(sites, None, strength=-80.0)
+        return T
+    A = make()
+    A.step(400)
+    A.save(tmp_path / "v3.bin")
+    raw = (tmp_path / "v3.bin").read_bytes()
+    assert raw[:8] == b"LQTTIS3\0"
+    nvox = A.nvox
+    pitch = (nvox + 255) // 256 * 256
+    head = 8 + 6 * 8 + 8
+    dref = np.frombuffer(raw, np.float64, 5, head)
+    o = head + 5 * 8
+    nr = 4 * 32 * pitch
+    r = raw[o:o + nr]
+    d = np.frombuffer(raw, np.float32, 5 * pitch, o + nr)
+    g = raw[o + nr + 4 * 5 * pitch:]
+    # chunk-major slow slots: chunk c, slot j, lane l at (c * 5 + j) * 64 + l
+    dd = d.astype(np.float64).reshape(-1, 5, 64) + dref[None, :, None]
+    old = b"LQTTIS2\0" + raw[8:head] + r + dd.astype(np.float64).tobytes() + g
+    (tmp_path / "v2.bin").write_bytes(old)
+    B = make()
+    B.load(tmp_path / "v2.bin")
+    assert B.time() == A.time()
+    for k in (0, 1, 63, 64, nvox - 1):
+        sa, sb = A.state(k), B.state(k)
+        rel = np.abs(sa - sb) / np.maximum(np.abs(sa), 1e-3)
+        assert rel.max() < 1e-6, (k, rel.max())
+    A.step(200)
+    B.step(200)
+    dv = np.max(np.abs(A.v() - B.v()))
+    print(f"earlier-format checkpoint: {nvox} voxels, max |dV| after 200 more steps {dv:.2e} mV")
+    assert dv < 1e-3
+
+
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
 def test_launch_shapes_agree_and_autotune(lqt, prec):
     """Every launch shape of the fused step (block size, XCD map, register
