@@ -522,7 +522,10 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
             for (int i = 0; i < nstrands;) {
                 int j = i;
                 while (j < nstrands && waves(o0[j]) == waves(o0[i])) ++j;
-                const bool split = mode == 3 && waves(o0[i]) <= 4;
+                // FP64 splits only strands of up to 2 wavefronts: a split
+                // workgroup of 6 or 8 wavefronts holds two per SIMD, and the
+                // FP64 update does not fit in their 256 registers each
+                const bool split = mode == 3 && waves(o0[i]) <= (s->fp32 ? 4 : 2);
                 const int b = 64 * waves(o0[i]) * (split ? 2 : 1);
                 s->layout[mode].push_back({b, grid_of(b, split, j - i), (int)s->hbund.size(), j - i, split});
                 for (int k = i; k < j; ++k) {

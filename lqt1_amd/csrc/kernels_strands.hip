@@ -22,8 +22,10 @@ __device__ __forceinline__ R wave_sum(R x) {
 }
 
 // FP32: two waves per SIMD (state, trackers and step constants fit in 256
-// registers).  FP64 keeps its registers (one wave per SIMD for blocks up to
-// 256 lanes) rather than spilling to scratch.
+// registers).  FP64 needs about 470 registers: blocks of up to 256 lanes
+// run one wave per SIMD and keep them; a strand of 5 to 8 wavefronts (257
+// to 512 cells) has two waves on some SIMDs and spills about half of its
+// registers to scratch, so its FP64 step is slower (it stays correct).
 //
 // SPLIT: every cell runs on two lanes of two wavefronts, which the
 // dispatcher places on different SIMDs: the first B/2 lanes advance the
@@ -302,11 +304,18 @@ __global__ void zero_int_kernel(int* p, int n) {
 template <class R>
 static void launch_b(int block, bool split, int grid, const StrandsArgs<R>& a, hipStream_t st) {
     if (split) {
-        switch (block) {
-            case 128: strands_kernel<R, 128, true><<<grid, 128, 0, st>>>(a); break;
-            case 256: strands_kernel<R, 256, true><<<grid, 256, 0, st>>>(a); break;
-            case 384: strands_kernel<R, 384, true><<<grid, 384, 0, st>>>(a); break;
-            default: strands_kernel<R, 512, true><<<grid, 512, 0, st>>>(a); break;
+        // FP64: split blocks of up to 256 lanes only (the host asks for no
+        // other); larger ones would hold two waves per SIMD and spill
+        if constexpr (sizeof(R) == 8) {
+            if (block == 128) strands_kernel<R, 128, true><<<grid, 128, 0, st>>>(a);
+            else strands_kernel<R, 256, true><<<grid, 256, 0, st>>>(a);
+        } else {
+            switch (block) {
+                case 128: strands_kernel<R, 128, true><<<grid, 128, 0, st>>>(a); break;
+                case 256: strands_kernel<R, 256, true><<<grid, 256, 0, st>>>(a); break;
+                case 384: strands_kernel<R, 384, true><<<grid, 384, 0, st>>>(a); break;
+                default: strands_kernel<R, 512, true><<<grid, 512, 0, st>>>(a); break;
+            }
         }
         return;
     }
@@ -343,7 +352,7 @@ int strands_occupancy(bool fp32, int block, bool split) {
         if (fp32) {
             switch (block) { OCC(float, 128, true) OCC(float, 256, true) OCC(float, 384, true) default: OCC(float, 512, true) }
         } else {
-            switch (block) { OCC(double, 128, true) OCC(double, 256, true) OCC(double, 384, true) default: OCC(double, 512, true) }
+            switch (block) { OCC(double, 128, true) default: OCC(double, 256, true) }
         }
     } else if (fp32) {
         switch (block) { OCC(float, 64, false) OCC(float, 128, false) OCC(float, 192, false) OCC(float, 256, false)
