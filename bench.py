@@ -11,12 +11,15 @@ plus the ghost refresh) with all state resident in HBM.  The headline value
 is FP32 cell-steps/s; the plain FP64 path is timed on the same geometry and
 steps, and the per-voxel activation and APD90 maps of the first beat in
 both precisions give the accuracy gate ("gate").  Further legs: the
-512-strand batched sweep (strands/s), the 2D annulus sheet, and the
+512-strand batched sweep (strands/s), the 2D annulus sheet, the
 surrogate-steered dose sweep against the plain batched sweep of the same
-grid ("surrogate"); the FP64 oracle on the host cores is timed beside them.
+grid ("surrogate"), and the same 3D step on a tissue the size of the
+reference's own whole-ventricle grid ("reference_grid": 690x489x537 cells
+at 0.2 mm, a 24.6M-voxel shell); the FP64 oracle on the host cores is timed
+beside them.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-strands]
-                    [--no-sheet] [--no-surrogate] [--no-accuracy3d]
+                    [--no-sheet] [--no-surrogate] [--no-accuracy3d] [--no-refgrid]
 """
 from __future__ import annotations
 
@@ -331,6 +334,35 @@ def strands_cpu(specs, nsample, threads):
                        f"beats, one strand per thread, {wall:.1f} s")
 
 
+def reference_grid_run(A, steps=20, warmup=5):
+    """The fused 3D step (default launch shape) on an ellipsoidal shell that
+    fills GPU3D.cu's 690 x 489 x 537 grid at 0.2 mm (24.6M voxels, the size
+    of the reference's whole-ventricle model), with the ventricle's
+    parameters (IKs x0.2, ISO on, anisotropic), FP32 and FP64."""
+    from lqt1_amd import geometry as G
+
+    t0 = time.perf_counter()
+    geo = G.ellipsoid_shell_3d()
+    out = dict(workload="ellipsoidal shell on the reference's 690x489x537 grid @0.2 mm, anisotropic D_f 0.154/"
+                        "D_c 0.0385, IKs x0.2 ISO on, apical stimulus", grid=[690, 489, 537],
+               geometry_s=round(time.perf_counter() - t0, 1))
+    for name, prec in (("fp32", A.FP32), ("fp64", A.FP64)):
+        T = A.Tissue(geo.label, geo.fibre, geo.dx, 0.154, 0.0385, A.preset("config3"), prec, 0.02)
+        T.set_stimulus(geo.stim_sites, geo.stim_offsets, bcl=1000.0, duration=3.0, strength=-52.0)
+        T.step(warmup)
+        A.lib().lqt_device_synchronize()
+        t0 = time.perf_counter()
+        T.step(steps)
+        wall = time.perf_counter() - t0
+        out["voxels"] = T.nvox
+        out[name] = dict(cell_steps_per_s=T.nvox * steps / wall, ms_per_step=wall / steps * 1e3,
+                         kernel_us=round(T.kernel_us()[0], 1), bytes_per_step=T.bytes_per_step(),
+                         guard=T.v_guard().tolist())
+        T.close()
+    out["fp32_over_fp64"] = out["fp32"]["cell_steps_per_s"] / out["fp64"]["cell_steps_per_s"]
+    return out
+
+
 def strands_throughput(A, nstr, nbeats, seed=0):
     """Batched strand sweep (strand_specs), one persistent launch."""
     import numpy as np
@@ -378,6 +410,8 @@ def main():
     ap.add_argument("--tune-steps", type=int, default=20, help="autotuner trial steps per launch shape (0 = off)")
     ap.add_argument("--no-strands", action="store_true")
     ap.add_argument("--no-sheet", action="store_true", help="skip the 2D annulus sheet leg")
+    ap.add_argument("--no-refgrid", action="store_true",
+                    help="skip the 3D step on the reference's 690x489x537 grid (24.6M voxels)")
     ap.add_argument("--no-accuracy3d", action="store_true",
                     help="skip the per-voxel FP32 vs FP64 activation / APD90 comparison over the first beat")
     ap.add_argument("--strand-beats", type=int, default=10, help="beats per strand of the 512-strand sweep")
@@ -534,6 +568,14 @@ def main():
         except Exception as e:
             sheet = dict(error=str(e))
 
+    refgrid = None
+    if not args.no_refgrid:
+        try:
+            refgrid = reference_grid_run(A)
+            log(f"reference grid: {refgrid}")
+        except Exception as e:
+            refgrid = dict(error=str(e))
+
     surrogate = None
     if not args.no_surrogate:
         try:
@@ -580,6 +622,7 @@ def main():
         "gate": gate,
         "strands": strands,
         "sheet2d": sheet,
+        "reference_grid": refgrid,
         "surrogate": surrogate,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -2,8 +2,9 @@
 
 The reference drivers read real geometries (GPU3D.cu:1276, new_11.cu:1296)
 that are not shipped with it; these generators give seeded stand-ins of the
-same kind: a 2D annular cross-section and a 3D biventricular shell with
-rotating fibres, transmural endo/M/epi labels and an M-cell island.
+same kind: a 2D annular cross-section, a 3D biventricular shell with
+rotating fibres, transmural endo/M/epi labels and an M-cell island, and an
+ellipsoidal shell that fills the reference's own 3D grid.
 Grid index is (x*ny + y)*nz + z; labels are enum CellType, NONE outside.
 """
 from __future__ import annotations
@@ -173,3 +174,47 @@ def ventricle_3d(shape=(256, 256, 320), dx=0.2, seed=0, lv=(10.5, 44.0, 20.5, 54
     sites = np.flatnonzero(stim).astype(np.int64)
     return Geometry(lab, fib, dx, sites, np.zeros(sites.size, np.float32), None,
                     depth.astype(np.float32), dict(island_centre_mm=ic.tolist(), seed=seed))
+
+
+def ellipsoid_shell_3d(shape=(690, 489, 537), dx=0.2, outer=(55.0, 42.0, 48.0), inner=(47.0, 34.0, 40.0),
+                       split=(0.30, 0.35, 0.35), fibre_endo=60.0, fibre_epi=-60.0, stim_depth=0.15,
+                       stim_z_mm=-30.0):
+    """An ellipsoidal shell centred in the grid (semi-axes `outer` and
+    `inner` in mm).  On the default grid, GPU3D.cu's 690 x 489 x 537 cells at
+    0.2 mm (GPU3D.cu:28-42), it holds 24.6M voxels, the size of the
+    reference's whole-ventricle model (about 22.6M).  Labels by the
+    transmural fraction u (0 endocardium, 1 epicardium) in the `split`
+    proportions; fibres circumferential about z, rotated towards z by an
+    angle going linearly from fibre_endo to fibre_epi; stimulus on the
+    endocardial voxels (u < stim_depth) below z = stim_z_mm.  Computed on
+    the tissue voxels only, so the full-size grid takes seconds."""
+    nx, ny, nz = shape
+    x = ((np.arange(nx, dtype=np.float32) - (nx - 1) / 2) * dx)[:, None, None]
+    y = ((np.arange(ny, dtype=np.float32) - (ny - 1) / 2) * dx)[None, :, None]
+    z = ((np.arange(nz, dtype=np.float32) - (nz - 1) / 2) * dx)[None, None, :]
+    ao, bo, co = outer
+    ai, bi, ci = inner
+    ro = (x / ao) ** 2 + (y / bo) ** 2 + (z / co) ** 2
+    ri = (x / ai) ** 2 + (y / bi) ** 2 + (z / ci) ** 2
+    idx = np.flatnonzero((ro <= 1.0) & (ri > 1.0))
+    del ro, ri
+    ix, iy, iz = np.unravel_index(idx, shape)
+    px = (ix.astype(np.float32) - (nx - 1) / 2) * dx
+    py = (iy.astype(np.float32) - (ny - 1) / 2) * dx
+    pz = (iz.astype(np.float32) - (nz - 1) / 2) * dx
+    so = np.sqrt((px / ao) ** 2 + (py / bo) ** 2 + (pz / co) ** 2)
+    si = np.sqrt((px / ai) ** 2 + (py / bi) ** 2 + (pz / ci) ** 2)
+    u = (si - 1.0) / np.maximum((si - 1.0) + (1.0 - so), 1e-6)
+    e1, e2 = split[0], split[0] + split[1]
+    lab = np.full(shape, L.NONE, np.uint8)
+    lab.reshape(-1)[idx] = np.where(u < e1, L.ENDO, np.where(u < e2, L.MCELL, L.EPI)).astype(np.uint8)
+    ang = np.deg2rad(fibre_endo + (fibre_epi - fibre_endo) * u)
+    rho = np.hypot(px, py) + 1e-6
+    fib = np.zeros(shape + (3,), np.float32)
+    f = fib.reshape(-1, 3)
+    f[idx, 0] = -np.cos(ang) * py / rho
+    f[idx, 1] = np.cos(ang) * px / rho
+    f[idx, 2] = np.sin(ang)
+    sites = idx[(u < stim_depth) & (pz < stim_z_mm)].astype(np.int64)
+    return Geometry(lab, fib, dx, sites, np.zeros(sites.size, np.float32), None, None,
+                    dict(voxels=int(idx.size), outer_mm=list(outer), inner_mm=list(inner)))

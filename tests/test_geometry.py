@@ -128,3 +128,24 @@ def test_every_sheet_stencil_neighbour_is_tissue_or_ghost_2d():
     for p in np.argwhere(geo.label[:, :, 0] != A.NONE):
         for o in ((1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (-1, -1), (1, -1), (-1, 1)):
             assert g[p[0] + o[0], p[1] + o[1], 0] != 0
+
+
+def test_ellipsoid_shell_on_a_small_grid():
+    """The reference-grid shell generator on a reduced grid: voxel count
+    against the analytic shell volume, labels in the split proportions by
+    depth, unit fibres tangent to the circumferential/long-axis frame, and
+    stimulus sites on the endocardial apex only."""
+    dx = 0.8
+    geo = G.ellipsoid_shell_3d(shape=(150, 112, 128), dx=dx)
+    lab, fib = geo.label, geo.fibre
+    t = lab != A.NONE
+    vol = 4.0 / 3.0 * np.pi * (55 * 42 * 48 - 47 * 34 * 40)
+    assert abs(t.sum() * dx ** 3 / vol - 1.0) < 0.03, t.sum() * dx ** 3 / vol
+    frac = [(lab == c).sum() / t.sum() for c in (A.ENDO, A.MCELL, A.EPI)]
+    assert all(f > 0.15 for f in frac), frac
+    n = np.linalg.norm(fib[t], axis=-1)
+    assert np.allclose(n, 1.0, atol=1e-5)
+    assert np.all(fib[~t] == 0)
+    st = np.unravel_index(geo.stim_sites, lab.shape)
+    assert geo.stim_sites.size > 0 and np.all(lab[st] == A.ENDO)
+    assert np.all((st[2] - (lab.shape[2] - 1) / 2) * dx < -30.0)
