@@ -110,6 +110,43 @@ def test_empty_inputs_are_refused(lqt):
                                     electrode_mm=-(n * 0.15))])
 
 
+def test_longest_strand_matches_oracle(lqt, oracle):
+    """The longest strand a handle takes, 512 cells: 8 wavefronts, a full
+    512-lane workgroup with two wavefronts per SIMD, where the FP64 update
+    spills part of its registers to scratch.  One beat of 600 ms in launch
+    layouts 0, 1 and 2: FP64 activation and APD90 of every cell to 1e-6 ms
+    and final V to 1e-6 mV against the oracle of OneD.cc's loop; FP32
+    activation within one step and APD90 within 1 ms of the oracle."""
+    O = oracle
+    spec = lqt.StrandSpec(lqt.transmural(512, 0.3, 0.35), lqt.preset("cpu_default", iks_scale=0.5), bcl=600.0,
+                          nbeats=1)
+    pp = spec.params
+    r = O.strand_run(spec.celltype, spec.coeff,
+                     O.params(iks_scale=pp.iks_scale, iso=pp.iso, pca_iso=pp.pca_iso, suprep=pp.suprep,
+                              dose_fmol=pp.dose_fmol),
+                     dx=spec.dx, bcl=spec.bcl, stop_time=spec.bcl, stim=spec.stim_strength,
+                     stim_dur=spec.stim_duration, stim_start=spec.stim_start, nstim=spec.nstim, record=False,
+                     act_beat=0)
+    assert np.all(r["act"] > 0), "the wave reaches the last cell"
+    for prec in (lqt.FP64, lqt.FP32):
+        for mode in (0, 1, 2):
+            st = lqt.Strands([spec], prec, 0.02)
+            st.set_mode(mode)
+            st.run()
+            act, apd, _ = st.cell_results(0)
+            fin = st.final_state(0)
+            st.close()
+            assert np.array_equal(apd >= 0, r["apd"] >= 0), (prec, mode)
+            ok = apd >= 0
+            if prec == lqt.FP64:
+                assert np.max(np.abs(act - r["act"])) < 1e-6, (mode, np.max(np.abs(act - r["act"])))
+                assert np.max(np.abs(apd[ok] - r["apd"][ok]), initial=0) < 1e-6, mode
+                assert np.max(np.abs(fin[:, 0] - r["state"][:, 0])) < 1e-6, mode
+            else:
+                assert np.max(np.abs(act - r["act"])) <= 0.02 + 1e-9, mode
+                assert np.max(np.abs(apd[ok] - r["apd"][ok]), initial=0) <= 1.0, mode
+
+
 def test_strand_time_tests_with_odd_protocol_match_oracle(lqt, oracle):
     """The loop's time tests arrive at the kernel as step indices evaluated on
     the host; with a BCL, stimulus start and duration that are not multiples
