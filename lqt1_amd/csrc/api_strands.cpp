@@ -342,9 +342,14 @@ static int bundle_waves(const Bundle& b, const std::vector<StrandDev>& d) {
 
 static void set_kmax(std::vector<Bundle>& bs, const std::vector<StrandDev>& d) {
     for (Bundle& b : bs) {
-        int64_t m = 0;
-        for (int i = 0; i < b.n; ++i) m = std::max(m, d[b.sidx[i]].nsteps);
+        int64_t m = 0, kb = INT64_MAX;
+        for (int i = 0; i < b.n; ++i) {
+            const StrandDev& h = d[b.sidx[i]];
+            m = std::max(m, h.nsteps);
+            kb = std::min({kb, h.k_mb0, h.k_wmax, h.k_rec0, h.k_vrec0});
+        }
         b.kmax = m + 2;
+        b.kbook = kb;
     }
 }
 
@@ -418,6 +423,9 @@ lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc* desc, const u
                   "strand %d: bad dx, bcl, stop_time, rec_every or nstim", i);
         LQT_CHECK(d.s2_duration >= 0 && d.s2_first >= 0 && d.s2_count >= 0, LQT_ERR_ARG,
                   "strand %d: bad S2 duration or cells", i);
+        // the kernel carries step indices and sample intervals in 32 bits
+        LQT_CHECK(d.stop_time / dt < (double)(1 << 30) && d.rec_every < (1 << 30) && d.vrec_every < (1 << 30),
+                  LQT_ERR_ARG, "strand %d: 2^30 steps or more (stop_time / dt), or a sample interval as long", i);
         // the pseudo-ECG weight 1/(x_j - x0)^2 needs the electrode off every cell
         bool on_cell = false;
         for (int j = 0; j < d.ncell && d.rec_every > 0; ++j) on_cell |= (j * d.dx - d.x0) * (j * d.dx - d.x0) == 0.0;

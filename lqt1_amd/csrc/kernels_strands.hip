@@ -14,6 +14,11 @@
 
 namespace lqt {
 
+template <bool B>
+struct BoolTag {
+    static constexpr bool value = B;
+};
+
 template <class R>
 __device__ __forceinline__ R wave_sum(R x) {
 #pragma unroll
@@ -60,7 +65,11 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
         __syncthreads();
         if (q >= A.nbundles) break;
         const Bundle& Q = A.bundles[q];
-        const int64_t kmax = Q.kmax;
+        // Step indices and sample counts are carried in 32 bits: the host
+        // refuses strands of 2^30 steps or more (and sample intervals of as
+        // many), so no index or increment below reaches INT32_MAX, which
+        // stands for "never" (the host's INT64_MAX).
+        const int kmax = (int)Q.kmax;
         const int slot = Q.wslot[wid];
         const bool has = slot >= 0;
         const int sidx = has ? Q.sidx[slot] : 0;
@@ -88,11 +97,6 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
         const R RD = (R)Di, RdD = (R)dDi;
         const R inv_dx = (R)(1.0 / dxd), inv_2dx = (R)(1.0 / (2.0 * dxd)), inv_dx2 = (R)(1.0 / (dxd * dxd));
         // pseudo-ECG weight of this cell: one-sided at the ends
-        R wphi = R(0);
-        if (live) {
-            const double r = (j * dxd - D.x0) * (j * dxd - D.x0);
-            wphi = (R)((j == 0 || j == n - 1) ? 1.0 / r : 1.0 / (2.0 * r));
-        }
         const int c = live ? A.cls[cell] : 0;
         const Consts<R> cc = A.consts[D.pidx];
 
@@ -106,25 +110,44 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
         s.nai = 7; s.ki = 145; s.cansr = 1.2; s.cajsr = 1.2; s.CaMKt = 0;
         R v = (R)A.v0[D.pidx];
 
-        CellTrack tr;
-        tr.act = -1; tr.rep = -1; tr.vrest = 0; tr.vmax = -1e30; tr.vmin = 1e30; tr.flags = 0;
-        // peak V after the S2 start (OneD_VW.cc:206-207,390-394); only
-        // cells 0 and n-1 report it
-        double wmax = -100.0;
         const bool s2_cell = j >= D.s2_first && j < D.s2_first + D.s2_count;
-
-        int64_t nrec = 0, nvrow = 0;
-        int pend = 0;          // a sample waits for its reduction
-        int fin_slot = -1;     // slot whose per-wave partials wait for the lead lane
-        int64_t fin_idx = 0;
-        int64_t pend_idx = 0;
-        // the next S1 range and sample steps (host-evaluated decisions)
-        int64_t s1i = 0, s1_on = INT64_MAX, s1_off = INT64_MAX;
-        if (has && D.s1_count > 0) {
-            s1_on = A.s1r[2 * D.s1_first];
-            s1_off = A.s1r[2 * D.s1_first + 1];
+        auto k32 = [](int64_t x) { return x > (int64_t)INT32_MAX ? INT32_MAX : (int)x; };
+        const int nsteps = k32(D.nsteps), k_s2a = k32(D.k_s2a), k_s2b = k32(D.k_s2b);
+        const int s1_first = (int)D.s1_first, s1_count = (int)D.s1_count;
+        // the next S1 range (host-evaluated decisions)
+        int s1i = 0, s1_on = INT32_MAX, s1_off = INT32_MAX;
+        if (has && s1_count > 0) {
+            s1_on = k32(A.s1r[2 * (int64_t)s1_first]);
+            s1_off = k32(A.s1r[2 * (int64_t)s1_first + 1]);
         }
-        int64_t next_rec = D.k_rec0, next_vrec = D.k_vrec0;
+        // Bookkeeping state: the beat tracker, the peak V after the S2 start
+        // (OneD_VW.cc:206-207,390-394; only cells 0 and n-1 report it), the
+        // pseudo-ECG weight of this cell (one-sided at the ends) and the
+        // sample counters and steps.  Only the steps from Q.kbook on use
+        // them; they are set just before those steps (book_init), so that
+        // nothing of them is held while the earlier steps run.
+        CellTrack tr;
+        double wmax;
+        R wphi;
+        int nrec, nvrow, fin_idx, pend_idx, next_rec, next_vrec;
+        int pend, fin_slot;
+        auto book_init = [&]() {
+            tr.act = -1; tr.rep = -1; tr.vrest = 0; tr.vmax = -1e30; tr.vmin = 1e30; tr.flags = 0;
+            wmax = -100.0;
+            wphi = R(0);
+            if (live) {
+                const double r = (j * dxd - D.x0) * (j * dxd - D.x0);
+                wphi = (R)((j == 0 || j == n - 1) ? 1.0 / r : 1.0 / (2.0 * r));
+            }
+            nrec = 0;
+            nvrow = 0;
+            pend = 0;       // a sample waits for its reduction
+            fin_slot = -1;  // slot whose per-wave partials wait for the lead lane
+            fin_idx = 0;
+            pend_idx = 0;
+            next_rec = k32(D.k_rec0);
+            next_vrec = k32(D.k_vrec0);
+        };
         double time = 0.0;
         int b = 0;
         // V^0 into LDS with mirrored ends
@@ -139,19 +162,26 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
         // slot b) are read after the barrier, by the diffusion term and the
         // epilogue.  vbuf, xa and xb alternate between two slots, so step
         // k+1 writes the slot nobody reads in step k.
-        for (int64_t k = 0; k < kmax; ++k, time += A.dt_d) {
-            const bool run = has && k < D.nsteps;
+        // The loop body as one step function of two forms: before the
+        // bundle's first bookkeeping step (Q.kbook: the earliest measured
+        // beat, pseudo-ECG or V-row sample or end-cell peak of its strands)
+        // the trackers and samples cannot fire and their code is left out
+        // (about 7 % of a step); from there on the full form runs.  Both
+        // forms do the same arithmetic on V and the state.
+        auto step = [&](int k, auto book) {
+            constexpr bool BOOK = decltype(book)::value;
+            const bool run = has && k < nsteps;
             if (k >= s1_off) {  // past this S1 range: load the next one
                 ++s1i;
-                s1_on = s1i < D.s1_count ? A.s1r[2 * (D.s1_first + s1i)] : INT64_MAX;
-                s1_off = s1i < D.s1_count ? A.s1r[2 * (D.s1_first + s1i) + 1] : INT64_MAX;
+                s1_on = s1i < s1_count ? k32(A.s1r[2 * (int64_t)(s1_first + s1i)]) : INT32_MAX;
+                s1_off = s1i < s1_count ? k32(A.s1r[2 * (int64_t)(s1_first + s1i) + 1]) : INT32_MAX;
             }
             const bool act = run && live;
             Pre<R> P;
             GroupA<R> GA;
             GroupB<R> GB;
             if (act) {
-                if (!SPLIT) {
+                if constexpr (!SPLIT) {
                     P = tp_pre(s, v, A.dt);
                     GA = tp_group_a(s, v, c, cc, A.dt, P);
                     GB = tp_group_b(s, v, c, cc, A.dt, P);
@@ -170,25 +200,27 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
                 }
             }
             __syncthreads();
-            // finish the sample whose per-wave partials were written last step
-            if (fin_slot >= 0 && lead) {
-                float tot = 0.f;
-                for (int w = 0; w < nws; ++w) tot += part[fin_slot][w0 + w];
-                A.phi[D.phi_off + fin_idx] = tot;
-            }
-            fin_slot = -1;
-            // reduce the sample recorded at the end of the previous step
-            if (pend) {
-                const R dv = live ? vbuf[b][p + 1] - vbuf[b][p - 1] : R(0);
-                const R tot = wave_sum<R>(-wphi * dv);
-                const int sl = (int)(pend_idx & 1);
-                if ((tid & 63) == 0) part[sl][wid] = (float)tot;
-                fin_slot = sl;
-                fin_idx = pend_idx;
-                pend = 0;
+            if constexpr (BOOK) {
+                // finish the sample whose per-wave partials were written last step
+                if (fin_slot >= 0 && lead) {
+                    float tot = 0.f;
+                    for (int w = 0; w < nws; ++w) tot += part[fin_slot][w0 + w];
+                    A.phi[D.phi_off + fin_idx] = tot;
+                }
+                fin_slot = -1;
+                // reduce the sample recorded at the end of the previous step
+                if (pend) {
+                    const R dv = live ? vbuf[b][p + 1] - vbuf[b][p - 1] : R(0);
+                    const R tot = wave_sum<R>(-wphi * dv);
+                    const int sl = (int)(pend_idx & 1);
+                    if ((tid & 63) == 0) part[sl][wid] = (float)tot;
+                    fin_slot = sl;
+                    fin_idx = pend_idx;
+                    pend = 0;
+                }
             }
             if (act) {
-                if (SPLIT) {
+                if constexpr (SPLIT) {
                     if (half == 0) {
                         GB.ICaL = xb[b][0][tid]; GB.ICaNa = xb[b][1][tid]; GB.ICaK = xb[b][2][tid];
                         GB.IKr = xb[b][3][tid]; GB.IK1 = xb[b][4][tid]; GB.INaCa_i = xb[b][5][tid];
@@ -206,17 +238,17 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
                 R ist = R(0);
                 if (k >= s1_on) {  // in-cycle time in the S1 window
                     if (j < D.nstim) ist = (R)D.stim_strength;
-                } else if (k >= D.k_s2a && k < D.k_s2b && s2_cell) {
+                } else if (k >= k_s2a && k < k_s2b && s2_cell) {
                     ist = (R)D.s2_strength;
                 }
                 v = tp_finish<R, false>(s, v, c, cc, A.dt, ist, dvgap, P, GA, GB, nullptr);
             }
-            if (run && out_half) {
+            if (BOOK && run && out_half) {  // trackers, peaks and samples
                 if (live) {
-                    if (k >= D.k_wmax && (double)v >= wmax) wmax = (double)v;
+                    if (k >= k32(D.k_wmax) && (double)v >= wmax) wmax = (double)v;
 
                     // beat tracker, in the measured beat only (floor(t/BCL) == k)
-                    if (k >= D.k_mb0 && k < D.k_mb1) {
+                    if (k >= k32(D.k_mb0) && k < k32(D.k_mb1)) {
                         const double vf = (double)v;
                         if (!(tr.flags & 1)) {
                             tr.flags |= 1;
@@ -248,7 +280,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
                 }
                 if (k == next_rec) {  // k % rec_every == 0 and time >= rec_from
                     next_rec += D.rec_every;
-                    if (nrec < D.phi_cap) {
+                    if (nrec < k32(D.phi_cap)) {
                         if (nrec == 0 && lead) A.phit0[sidx] = time;
                         pend = 1;
                         pend_idx = nrec++;
@@ -256,7 +288,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
                 }
                 if (k == next_vrec) {
                     next_vrec += D.vrec_every;
-                    if (nvrow < D.vrec_cap) {
+                    if (nvrow < k32(D.vrec_cap)) {
                         if (live) A.vrec[D.vrec_off + nvrow * (int64_t)n + j] = (double)v;
                         if (nvrow == 0 && lead) A.vrect0[sidx] = time;
                         ++nvrow;
@@ -270,7 +302,12 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 
                 if (j == n - 1) vbuf[b ^ 1][seg + n + 1] = v;
             }
             b ^= 1;
-        }
+        };
+        const int kb = Q.kbook < (int64_t)kmax ? (int)Q.kbook : kmax;
+        int k = 0;
+        for (; k < kb; ++k, time += A.dt_d) step(k, BoolTag<false>{});
+        book_init();
+        for (; k < kmax; ++k, time += A.dt_d) step(k, BoolTag<true>{});
 
         if (live) {
             if (out_half) {

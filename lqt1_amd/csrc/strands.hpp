@@ -40,6 +40,10 @@ struct Bundle {
     int wave0[MAXS];
     int wslot[MAXS];  // strand slot of wavefront w, -1 if unused
     int64_t kmax;
+    // first step at which any of its strands can track, sample or record:
+    // min over them of k_mb0, k_wmax, k_rec0, k_vrec0 (the kernel leaves
+    // that code out of the steps before it)
+    int64_t kbook;
 };
 
 // Per-cell beat tracker: -30 mV activation (OneD.cc:422), running peak,
