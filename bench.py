@@ -471,6 +471,19 @@ def main():
                 us = st.last_ms() * 1e3 / (1000.0 / 0.02)
                 st.close()
                 log(f"strand-latency: {n} cells, {(n + 63) // 64} wavefronts, layout {mode}: {us:.2f} us per step")
+        # the steered sweep's re-check strand: 100 cells, 10 beats at BCL
+        # 1000 ms, its QT and EAD flag measured in the last beat only
+        from lqt1_amd import surrogate as S
+
+        spec = S.spec_of(S.Point(0.3, 1, 0.5, 1000.0, 100, 0.35), 10)
+        for mode in (0, 3):
+            st = A.Strands([spec], A.FP32, 0.02)
+            st.set_mode(mode)
+            st.run()
+            st.run()
+            us = st.last_ms() * 1e3 / (10 * 1000.0 / 0.02)
+            st.close()
+            log(f"strand-latency: re-check strand (100 cells, 10 beats), layout {mode}: {us:.2f} us per step")
         return
     if args.profile == "cells":
         # the ionic update alone on the ventricle's voxel count and type mix
