@@ -430,28 +430,58 @@ lqt_status lqt_cells_pace(lqt_cells* c, const lqt_pacing* pace, double* stats, i
     a.stim_start = pace->stim_start;
     a.stim_dur = pace->stim_duration;
     a.stim_strength = pace->stim_strength;
-    a.numS1 = pace->nbeats;
     a.dt = c->dt;
     a.trace_cell = trace_cell;
-    a.trace_every = trace_every;
+    a.trace_every = vtrace ? trace_every : 0;
     a.trace_cap = trace_cap;
     a.trace = vtrace ? c->trace : nullptr;
-    const double stopTime = a.numS1 * a.bcl;
+    const double numS1 = pace->nbeats;  // SingleCell.cc:45 compares against a double
+    const double bcl = a.bcl, s0 = a.stim_start, sd = a.stim_dur, ddt = c->dt;
+    const double stopTime = numS1 * bcl;
     // SingleCell.cc:95: for (time = 0; time < stopTime; time += dt), split
-    // into launches of at most `chunk` steps; the host replays the same
-    // double accumulator to give each launch its start time.
+    // into launches of at most `chunk` steps.  The host replays the same
+    // double accumulator and evaluates the loop's time tests with the
+    // reference's expressions (SingleCell.cc:98-106 stimulus window,
+    // :144-147 start of the last beat's statistics, :149 the last beat),
+    // handing each launch its steps as integers.  A launch also ends before
+    // a (kPaceRanges + 1)-th stimulus window would open in it.
     const int chunk = 20000;
     double time = 0.0;
     int64_t step = 0;
     while (time < stopTime) {
+        a.time0 = time;
+        a.nstim = 0;
+        a.rest_step = -1;
+        a.last0 = a.last1 = 0;
+        a.trace_next = INT32_MAX;
+        a.trace_slot = 0;
         double t = time;
         int k = 0;
+        bool prev_on = false, in_last = false;
         while (k < chunk && t < stopTime) {
-            t += c->dt;
+            const double tc = t - floor(t / bcl) * bcl;
+            const bool on = tc >= s0 && tc < s0 + sd;
+            if (on && !prev_on) {
+                if (a.nstim == kPaceRanges) break;
+                a.stim_on[a.nstim] = k;
+                a.stim_off[a.nstim] = chunk;
+                ++a.nstim;
+            }
+            if (!on && prev_on) a.stim_off[a.nstim - 1] = k;
+            prev_on = on;
+            if (floor((t - s0) / bcl) == numS1 - 2 && floor((t + ddt - s0) / bcl) == numS1 - 1) a.rest_step = k;
+            const bool last = floor((t - s0) / bcl) == numS1 - 1;
+            if (last && !in_last) a.last0 = k;
+            if (last) a.last1 = k + 1;
+            in_last = last;
+            if (a.trace_every > 0 && a.trace_next == INT32_MAX && (step + k) % a.trace_every == 0) {
+                a.trace_next = k;
+                a.trace_slot = (step + k) / a.trace_every;
+            }
+            t += ddt;
             ++k;
         }
-        a.time0 = time;
-        a.step0 = step;
+        if (prev_on) a.stim_off[a.nstim - 1] = k;
         a.nsteps = k;
         launch_cells_pace(c->fp32, c->n, c->r, c->d, c->pitch, c->v, c->cls, c->pidx, c->consts, c->dt, c->track, a,
                           c->stream);

@@ -19,14 +19,23 @@ struct PaceTrack {
     int pad;
 };
 
+// One launch of the pacing protocol.  Its time tests (SingleCell.cc:95-215)
+// arrive as step indices of this launch that the host evaluated with the
+// protocol's own expressions on the same double time accumulator, so the
+// kernel compares integers instead of dividing in double every step.
+constexpr int kPaceRanges = 4;
 struct PaceArgs {
     double bcl, stim_start, stim_dur, stim_strength;
-    double numS1;       // nbeats as double (SingleCell.cc:45)
     double dt;          // step of the double time accumulator
     double time0;       // time at the first step of this launch
-    int64_t step0;      // index of the first step of this launch
     int nsteps;
+    int nstim;          // stimulus on in steps [stim_on[r], stim_off[r]), r < nstim
+    int stim_on[kPaceRanges], stim_off[kPaceRanges];
+    int rest_step;      // the step at which the last beat's statistics start, or -1
+    int last0, last1;   // steps [last0, last1) lie in the last beat
     int trace_cell, trace_every;
+    int trace_next;     // first step of this launch that records the trace, or INT32_MAX
+    int64_t trace_slot; // trace slot of that step
     int64_t trace_cap;
     double* trace;      // device, or nullptr
 };

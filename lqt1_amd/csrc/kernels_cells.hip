@@ -27,9 +27,9 @@ __global__ __launch_bounds__(256) void cells_step_kernel(int n, SoA<R> st, R* __
     store_state(s, st, k);
 }
 
-// The pacing protocol of SingleCell.cc:95-215 for every cell: stimulus by
-// the in-cycle time of the double time accumulator, then the last-beat
-// statistics (rest, dV/dt max, overshoot, APDx).
+// The pacing protocol of SingleCell.cc:95-215 for every cell: stimulus in
+// the host-evaluated step ranges of the in-cycle time window, then the
+// last-beat statistics (rest, dV/dt max, overshoot, APDx).
 template <class R>
 __global__ __launch_bounds__(256) void cells_pace_kernel(int n, SoA<R> st, R* __restrict__ v,
                                                          const uint8_t* __restrict__ cls,
@@ -47,32 +47,34 @@ __global__ __launch_bounds__(256) void cells_pace_kernel(int n, SoA<R> st, R* __
     double time = a.time0;
     const double ddt = a.dt;
     static const double frac[6] = {0.20, 0.25, 0.50, 0.75, 0.80, 0.90};
+    int r = 0;
+    int tnext = a.trace_next;
+    int64_t tslot = a.trace_slot;
     for (int i = 0; i < a.nsteps; ++i, time += ddt) {
-        const double tc = time - floor(time / a.bcl) * a.bcl;
-        const R Ist = (tc >= a.stim_start && tc < a.stim_start + a.stim_dur) ? R(a.stim_strength) : R(0);
+        while (r < a.nstim && i >= a.stim_off[r]) ++r;
+        const R Ist = (r < a.nstim && i >= a.stim_on[r]) ? R(a.stim_strength) : R(0);
         Diag dg;
         vk = tpord_update<R, true>(s, vk, c, cc, dt, Ist, R(0), &dg);
         const double V = (double)vk;
-        const int64_t step = a.step0 + i;
-        if (a.trace && k == a.trace_cell && a.trace_every > 0 && step % a.trace_every == 0) {
-            const int64_t slot = step / a.trace_every;
-            if (slot < a.trace_cap) {
-                a.trace[3 * slot] = V;
-                a.trace[3 * slot + 1] = dg.ICaL;
-                a.trace[3 * slot + 2] = dg.IKs;
+        if (i == tnext) {
+            if (a.trace && k == a.trace_cell && tslot < a.trace_cap) {
+                a.trace[3 * tslot] = V;
+                a.trace[3 * tslot + 1] = dg.ICaL;
+                a.trace[3 * tslot + 2] = dg.IKs;
             }
+            tnext += a.trace_every;
+            ++tslot;
         }
-        const double absdvdt = fabs(dg.dV_dt);
-        if (floor((time - a.stim_start) / a.bcl) == a.numS1 - 2 &&
-            floor((time + ddt - a.stim_start) / a.bcl) == a.numS1 - 1) {
+        if (i == a.rest_step) {
             tr.rest = V;
             tr.oldV = V;
             tr.peakfound = 0;
             tr.dvdt_max = -1;
         }
-        if (floor((time - a.stim_start) / a.bcl) == a.numS1 - 1) {
+        if (i >= a.last0 && i < a.last1) {
             if (V > tr.oldV) {
                 tr.oldV = V;
+                const double absdvdt = fabs(dg.dV_dt);
                 if (absdvdt > tr.dvdt_max) tr.dvdt_max = absdvdt;
             } else if (!tr.peakfound) {
                 tr.peakfound = 1;
@@ -81,10 +83,9 @@ __global__ __launch_bounds__(256) void cells_pace_kernel(int n, SoA<R> st, R* __
                 for (int j = 0; j < 6; ++j) tr.repo[j] = tr.overshoot - frac[j] * tr.amplitude;
                 tr.oldV = V;
             } else {
-                const double tcs = time - floor(time / a.bcl) * a.bcl - a.stim_start - a.stim_dur;
                 for (int j = 0; j < 6; ++j) {
                     if (tr.oldV >= tr.repo[j] && V <= tr.repo[j]) {
-                        tr.apd[j] = tcs;
+                        tr.apd[j] = time - floor(time / a.bcl) * a.bcl - a.stim_start - a.stim_dur;
                         break;
                     }
                 }
