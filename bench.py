@@ -429,7 +429,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (one GPU per box, "
-              "no multi-GPU path in this build)", file=sys.stderr)
+              "no multi-GPU path in this library)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

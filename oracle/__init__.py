@@ -2,7 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and
 the ``cpu_baseline`` leg of ``bench.py`` import this package, and only as the
-checker.  The product package ``lqt1_amd`` never imports it.
+reference that results are compared with.  The product package ``lqt1_amd``
+never imports it.
 
 Every function restates the reference CPU path (see ``oracle.h`` for the
 file:line map); ``ref_*`` helpers load the reference's own TPORd class from

@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links or calls
  * this code; it is loaded by tests/, by the smoke check and by the CPU
- * baseline leg of the benchmark, always as the checker.
+ * baseline leg of the benchmark, always as the reference that results are
+ * compared with.
  *
  * It restates, in plain C, the arithmetic of the reference's CPU path:
  *   SingleCell/TPORd.cc:142-198   (init)      -> or_cell_init
@@ -88,7 +89,7 @@ int64_t or_strand_run(int n, const uint8_t *celltype, const double *coeff,
 
 /* Per-cell beat measurements inside beat k = floor(t/BCL) (the reference
  * defines only the -30 mV activation, OneD.cc:422; APD90 and the EAD test
- * are this build's definitions, restated here independently of the
+ * are the library's own definitions, restated here independently of the
  * kernel):  rest = V at the first step of the beat; act = first t with
  * V >= -30 mV; vmax = running peak after act; repolarised at the first t
  * with V <= vmax - 0.9 (vmax - rest), APD90 = t_rep - act; EAD when, from

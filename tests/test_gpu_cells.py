@@ -3,7 +3,7 @@
 Tolerances: FP64 kernels differ from the oracle only by the device libm
 (about 1 ulp per exp/log) and by the rearranged algebra (exp(s*v+o),
 x/expm1(x), rate forms of tau), so states agree to 1e-9 relative after
-1000 steps and V to 1e-6 mV over a whole beat.  The FP32 kernel is judged
+1000 steps and V to 1e-6 mV over a whole beat.  The FP32 kernel is checked
 as a solver: V L-inf over a paced beat and APD90 within 1 ms.
 """
 import numpy as np
