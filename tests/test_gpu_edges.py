@@ -101,6 +101,10 @@ def test_empty_inputs_are_refused(lqt):
     for n in (1, 513):
         with pytest.raises(lqt.LqtError):
             lqt.Strands([lqt.StrandSpec(lqt.transmural(n), lqt.preset("cpu_default"), nbeats=1)])
+    # the strand kernel carries step indices in 32 bits: 2^30 steps are refused
+    with pytest.raises(lqt.LqtError, match="2\\^30"):
+        lqt.Strands([lqt.StrandSpec(lqt.transmural(20), lqt.preset("cpu_default"), nbeats=1,
+                                    stop_time=(1 << 30) * 0.02 + 1.0)])
     with pytest.raises(lqt.LqtError):
         lqt.Cells(np.zeros(0, np.uint8))
     # a pseudo-ECG electrode on a cell (here cell 0) would weight it by 1/0
