@@ -303,7 +303,9 @@ typedef struct {
 
 typedef struct lqt_strands lqt_strands;
 
-/* celltype/coeff indexed by desc.cell_offset + i; coeff may be NULL (0.154). */
+/* celltype/coeff indexed by desc.cell_offset + i; coeff may be NULL (0.154).
+ * A strand has 2..512 cells and fewer than 2^30 steps (stop_time / dt), and
+ * its sample intervals are below 2^30 steps; LQT_ERR_ARG otherwise. */
 lqt_status lqt_strands_create(int nstrands, const lqt_strand_desc *desc, const uint8_t *celltype,
                               const double *coeff, const lqt_params *params, int nparams,
                               lqt_precision prec, double dt, lqt_strands **out);
