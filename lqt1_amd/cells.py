@@ -38,6 +38,17 @@ class Cells:
         dvg = None if dvgap is None else np.ascontiguousarray(np.broadcast_to(dvgap, (self.n,)), np.float64)
         L.check(L.lib().lqt_cells_step(self.h, int(nsteps), L.ptr(ist), L.ptr(dvg)))
 
+    @property
+    def split(self):
+        """True if the next step or pace call runs each cell on two lanes."""
+        v = C.c_int(0)
+        L.check(L.lib().lqt_cells_get_split(self.h, C.byref(v)))
+        return bool(v.value)
+
+    def set_split(self, mode):
+        """Lane layout: 1 two lanes per cell, 0 one lane, -1 automatic."""
+        L.check(L.lib().lqt_cells_set_split(self.h, int(mode)))
+
     def state(self):
         out = np.zeros((self.n, NSTATE))
         L.check(L.lib().lqt_cells_get_state(self.h, L.ptr(out)))

@@ -194,9 +194,26 @@ struct lqt_cells {
     double* trace = nullptr;
     int64_t trace_cap = 0;
     hipStream_t stream = nullptr;
+    int split_mode = -1;       // lqt_cells_set_split
+    int ncu = 0;               // compute units of the device
 };
 
 static size_t rsize(const lqt_cells* c) { return c->fp32 ? sizeof(float) : sizeof(double); }
+
+// Split cells (two lanes each) while their wavefronts run one per SIMD.  An
+// FP64 split wavefront needs more than half of a SIMD's registers, so the
+// four SIMDs of a compute unit take two split workgroups, one wavefront on
+// each SIMD; FP32 ones fit two or three to a SIMD, and a compute unit given
+// two split workgroups can place them on the same two SIMDs (measured: a
+// step 1.2x the single-lane one at 512 workgroups on 256 compute units), so
+// FP32 splits while there is at most one workgroup per compute unit.
+// Beyond that every SIMD is busy either way and the single-lane update,
+// which issues fewer instructions in total, is faster.
+static bool use_split(const lqt_cells* c) {
+    if (c->split_mode >= 0) return c->split_mode == 1;
+    const int64_t groups = ((int64_t)c->n + 63) / 64;
+    return groups <= (c->fp32 ? 1 : 2) * (int64_t)c->ncu;
+}
 
 extern "C" {
 
@@ -213,8 +230,13 @@ lqt_status lqt_cells_create(int n, const uint8_t* celltype, const lqt_params* pa
     int ndev = 0;
     LQT_CHECK(hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0, LQT_ERR_HIP,
               "lqt_cells_create: no HIP device available");
+    int dev = 0, ncu = 0;
+    LQT_CHECK(hipGetDevice(&dev) == hipSuccess &&
+                  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0,
+              LQT_ERR_HIP, "lqt_cells_create: cannot read the device's compute unit count");
     lqt_cells* c = new lqt_cells();
     c->n = n;
+    c->ncu = ncu;
     c->pitch = ((int64_t)n + 63) / 64 * 64;
     c->fp32 = prec == LQT_FP32;
     c->dt = dt;
@@ -294,18 +316,30 @@ lqt_status lqt_cells_step(lqt_cells* c, int nsteps, const double* istim, const d
         dg = c->dbuf + c->pitch;
         LQT_HIP(hipMemcpyAsync(dg, dvgap, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
     }
-    launch_cells_step(c->fp32, c->n, c->r, c->d, c->pitch, c->v, c->cls, c->pidx, c->consts, nsteps, c->dt, di,
-                      dg, c->stream);
+    launch_cells_step(c->fp32, use_split(c), c->n, c->r, c->d, c->pitch, c->v, c->cls, c->pidx, c->consts, nsteps,
+                      c->dt, di, dg, c->stream);
     LQT_HIP(hipGetLastError());
     LQT_HIP(hipStreamSynchronize(c->stream));
+    return LQT_OK;
+}
+
+lqt_status lqt_cells_set_split(lqt_cells* c, int mode) {
+    LQT_CHECK(c && mode >= -1 && mode <= 1, LQT_ERR_ARG, "lqt_cells_set_split: mode must be -1, 0 or 1");
+    c->split_mode = mode;
+    return LQT_OK;
+}
+
+lqt_status lqt_cells_get_split(const lqt_cells* c, int* split) {
+    LQT_CHECK(c && split, LQT_ERR_ARG, "lqt_cells_get_split: bad argument");
+    *split = use_split(c) ? 1 : 0;
     return LQT_OK;
 }
 
 lqt_status lqt_cells_step_async(lqt_cells* c, int nsteps, const double* istim, const double* dvgap, void* stream) {
     LQT_CHECK(c && nsteps >= 0, LQT_ERR_ARG, "lqt_cells_step_async: bad argument");
     if (nsteps == 0) return LQT_OK;
-    launch_cells_step(c->fp32, c->n, c->r, c->d, c->pitch, c->v, c->cls, c->pidx, c->consts, nsteps, c->dt, istim,
-                      dvgap, stream ? (hipStream_t)stream : c->stream);
+    launch_cells_step(c->fp32, use_split(c), c->n, c->r, c->d, c->pitch, c->v, c->cls, c->pidx, c->consts, nsteps,
+                      c->dt, istim, dvgap, stream ? (hipStream_t)stream : c->stream);
     LQT_HIP(hipGetLastError());
     return LQT_OK;
 }
@@ -483,8 +517,8 @@ lqt_status lqt_cells_pace(lqt_cells* c, const lqt_pacing* pace, double* stats, i
         }
         if (prev_on) a.stim_off[a.nstim - 1] = k;
         a.nsteps = k;
-        launch_cells_pace(c->fp32, c->n, c->r, c->d, c->pitch, c->v, c->cls, c->pidx, c->consts, c->dt, c->track, a,
-                          c->stream);
+        launch_cells_pace(c->fp32, use_split(c), c->n, c->r, c->d, c->pitch, c->v, c->cls, c->pidx, c->consts,
+                          c->dt, c->track, a, c->stream);
         LQT_HIP(hipGetLastError());
         time = t;
         step += k;
