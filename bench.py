@@ -425,7 +425,7 @@ def main():
                     help="short run of one kernel for rocprofv3 (no JSON line); cells = the ionic update "
                          "alone, --steps updates per launch with the state in registers; cell-latency = "
                          "time per step of the single-cell pacing protocol (2 beats at BCL 1000 ms) by cell "
-                         "count, one lane and two lanes per cell; strands = the "
+                         "count, one, two and four lanes per cell; strands = the "
                          "512-strand sweep of --strand-beats beats; strand-latency = time per step of one "
                          "strand alone on the GPU by length (one beat at BCL 1000 ms); sheet = the 2D annulus "
                          "leg alone")
@@ -490,18 +490,18 @@ def main():
         return
     if args.profile == "cell-latency":
         # the SingleCell.cc protocol on few cells: a step costs one lane's
-        # dependent chain, which the split layout (two lanes per cell)
-        # shortens; 1 and 16 cells leave most lanes of their wavefront idle
+        # dependent chain, which two or four lanes per cell shorten; 1 and
+        # 16 cells leave most lanes of their wavefront idle
         for prec, pname in ((A.FP32, "fp32"), (A.FP64, "fp64")):
             for n in (1, 16, 17, 64, 4096):
-                for split in (0, 1):
+                for lanes in (1, 2, 4):
                     c = A.Cells(np.full(n, A.MCELL, np.uint8), A.preset("cpu_default"), prec, 0.02)
-                    c.set_split(split)
+                    c.set_lanes(lanes)
                     t0 = time.perf_counter()
                     st, _ = c.pace(bcl=1000.0, nbeats=2)
                     us = (time.perf_counter() - t0) / (2 * 1000.0 / 0.02) * 1e6
                     c.close()
-                    log(f"cell-latency: {pname}, {n} cells, {'two lanes' if split else 'one lane'} per cell: "
+                    log(f"cell-latency: {pname}, {n} cells, {lanes} lane{'s' if lanes > 1 else ''} per cell: "
                         f"{us:.2f} us per step, APD90 {st[0, 9]:.2f} ms")
         return
     if args.profile == "cells":

@@ -74,7 +74,7 @@ lqt_status lqt_device_name(char *buf, int len);
 lqt_status lqt_bench_copy(int64_t bytes, int reps, double *us_per_copy);
 
 /* ---------------------------------------------------------------------------
- * Cells: n independent TPORd cells on the GPU, one or two lanes per cell.
+ * Cells: n independent TPORd cells on the GPU, one, two or four lanes per cell.
  * Replaces the Cell/TPORd object interface (SingleCell/Cell.cc:22-42,
  * SingleCell/TPORd.cc:137-376) used by SingleCell.cc and OneD.cc:
  * setIstim / setDVgap_dt / update / getV / outputAllStates / readinAllStates.
@@ -95,17 +95,17 @@ lqt_status lqt_cells_step(lqt_cells *c, int nsteps, const double *istim, const d
  * other calls use its own stream: the caller synchronises `stream` before
  * reading states back or stepping again from another stream. */
 lqt_status lqt_cells_step_async(lqt_cells *c, int nsteps, const double *istim, const double *dvgap, void *stream);
-/* Lane layout of lqt_cells_step / lqt_cells_step_async / lqt_cells_pace
+/* Lanes per cell of lqt_cells_step / lqt_cells_step_async / lqt_cells_pace
  * (no counterpart in the reference, whose cells are objects stepped one
- * after another): mode 1 runs each cell on two lanes of two wavefronts, one
- * per half of the ionic currents, which exchange their currents every step
- * (same results bit for bit, about 0.6 of the dependent chain per step);
- * mode 0 one lane per cell; mode -1 (the default) splits while the split
- * cells' wavefronts run one per SIMD: n <= 64 x compute units in FP32
- * (16,384 on MI355X), 128 x compute units in FP64 (32,768).
- * lqt_cells_get_split reports the layout that the next call uses. */
-lqt_status lqt_cells_set_split(lqt_cells *c, int mode);
-lqt_status lqt_cells_get_split(const lqt_cells *c, int *split);
+ * after another): 2 or 4 lanes of as many wavefronts share a cell's ionic
+ * currents (two groups, or each group in two parts) and exchange them every
+ * step, with the same results bit for bit and a shorter dependent chain per
+ * step; 1 runs one lane per cell; 0 (the default) chooses by size: 4 while
+ * ceil(n / 64) <= compute units (n <= 16,384 on MI355X) and in FP32 up to
+ * twice that, 2 in FP64 up to twice that, else 1.  lqt_cells_get_lanes
+ * reports what the next call uses. */
+lqt_status lqt_cells_set_lanes(lqt_cells *c, int lanes);
+lqt_status lqt_cells_get_lanes(const lqt_cells *c, int *lanes);
 /* All 40 states per cell in the order of TPORd::outputAllStates
  * (TPORd.cc:200-252), out[n*40]. */
 lqt_status lqt_cells_get_state(lqt_cells *c, double *out);

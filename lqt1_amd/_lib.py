@@ -129,8 +129,8 @@ def lib():
     _sig(L, "lqt_cells_destroy", [_VP])
     _sig(L, "lqt_cells_step", [_VP, C.c_int, _VP, _VP])
     _sig(L, "lqt_cells_step_async", [_VP, C.c_int, _VP, _VP, _VP])
-    _sig(L, "lqt_cells_set_split", [_VP, C.c_int])
-    _sig(L, "lqt_cells_get_split", [_VP, P(C.c_int)])
+    _sig(L, "lqt_cells_set_lanes", [_VP, C.c_int])
+    _sig(L, "lqt_cells_get_lanes", [_VP, P(C.c_int)])
     _sig(L, "lqt_cells_get_state", [_VP, _VP])
     _sig(L, "lqt_cells_set_state", [_VP, _VP])
     _sig(L, "lqt_cells_get_v", [_VP, _VP])

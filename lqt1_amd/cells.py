@@ -39,15 +39,15 @@ class Cells:
         L.check(L.lib().lqt_cells_step(self.h, int(nsteps), L.ptr(ist), L.ptr(dvg)))
 
     @property
-    def split(self):
-        """True if the next step or pace call runs each cell on two lanes."""
+    def lanes(self):
+        """Lanes per cell (1, 2 or 4) of the next step or pace call."""
         v = C.c_int(0)
-        L.check(L.lib().lqt_cells_get_split(self.h, C.byref(v)))
-        return bool(v.value)
+        L.check(L.lib().lqt_cells_get_lanes(self.h, C.byref(v)))
+        return v.value
 
-    def set_split(self, mode):
-        """Lane layout: 1 two lanes per cell, 0 one lane, -1 automatic."""
-        L.check(L.lib().lqt_cells_set_split(self.h, int(mode)))
+    def set_lanes(self, lanes):
+        """Lanes per cell: 1, 2 or 4, or 0 for the automatic choice."""
+        L.check(L.lib().lqt_cells_set_lanes(self.h, int(lanes)))
 
     def state(self):
         out = np.zeros((self.n, NSTATE))

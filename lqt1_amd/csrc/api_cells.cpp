@@ -194,25 +194,30 @@ struct lqt_cells {
     double* trace = nullptr;
     int64_t trace_cap = 0;
     hipStream_t stream = nullptr;
-    int split_mode = -1;       // lqt_cells_set_split
+    int lanes = 0;             // lqt_cells_set_lanes (0: automatic)
     int ncu = 0;               // compute units of the device
 };
 
 static size_t rsize(const lqt_cells* c) { return c->fp32 ? sizeof(float) : sizeof(double); }
 
-// Split cells (two lanes each) while their wavefronts run one per SIMD.  An
-// FP64 split wavefront needs more than half of a SIMD's registers, so the
-// four SIMDs of a compute unit take two split workgroups, one wavefront on
-// each SIMD; FP32 ones fit two or three to a SIMD, and a compute unit given
-// two split workgroups can place them on the same two SIMDs (measured: a
-// step 1.2x the single-lane one at 512 workgroups on 256 compute units), so
-// FP32 splits while there is at most one workgroup per compute unit.
-// Beyond that every SIMD is busy either way and the single-lane update,
-// which issues fewer instructions in total, is faster.
-static bool use_split(const lqt_cells* c) {
-    if (c->split_mode >= 0) return c->split_mode == 1;
+// Lanes per cell of the next launch.  More lanes per cell shorten the
+// step while the wavefronts do not crowd the SIMDs (measured with the
+// single-cell update at 8,192-49,152 cells on 256 compute units): four
+// lanes, a 256-lane workgroup of four wavefronts per 64 cells, up to one
+// workgroup per compute unit, and in FP32 up to two (two four-lane
+// wavefronts on a SIMD still issue fewer instructions than one single-lane
+// wavefront: 1.71 against 2.15 us per step); an FP64 four-lane wavefront
+// needs more than half of a SIMD's registers, so a second workgroup waits
+// for the first (7.88 us) and two lanes (6.2 us against 10.0 on one) are
+// used up to two 128-lane workgroups per compute unit.  Beyond that every
+// SIMD is busy either way and the single-lane update, which issues the
+// fewest instructions in total, is fastest.
+static int lanes_per_cell(const lqt_cells* c) {
+    if (c->lanes > 0) return c->lanes;
     const int64_t groups = ((int64_t)c->n + 63) / 64;
-    return groups <= (c->fp32 ? 1 : 2) * (int64_t)c->ncu;
+    if (groups <= (int64_t)c->ncu) return 4;
+    if (groups <= 2 * (int64_t)c->ncu) return c->fp32 ? 4 : 2;
+    return 1;
 }
 
 extern "C" {
@@ -316,30 +321,31 @@ lqt_status lqt_cells_step(lqt_cells* c, int nsteps, const double* istim, const d
         dg = c->dbuf + c->pitch;
         LQT_HIP(hipMemcpyAsync(dg, dvgap, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
     }
-    launch_cells_step(c->fp32, use_split(c), c->n, c->r, c->d, c->pitch, c->v, c->cls, c->pidx, c->consts, nsteps,
-                      c->dt, di, dg, c->stream);
+    launch_cells_step(c->fp32, lanes_per_cell(c), c->n, c->r, c->d, c->pitch, c->v, c->cls, c->pidx, c->consts,
+                      nsteps, c->dt, di, dg, c->stream);
     LQT_HIP(hipGetLastError());
     LQT_HIP(hipStreamSynchronize(c->stream));
     return LQT_OK;
 }
 
-lqt_status lqt_cells_set_split(lqt_cells* c, int mode) {
-    LQT_CHECK(c && mode >= -1 && mode <= 1, LQT_ERR_ARG, "lqt_cells_set_split: mode must be -1, 0 or 1");
-    c->split_mode = mode;
+lqt_status lqt_cells_set_lanes(lqt_cells* c, int lanes) {
+    LQT_CHECK(c && (lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4), LQT_ERR_ARG,
+              "lqt_cells_set_lanes: lanes must be 0 (automatic), 1, 2 or 4");
+    c->lanes = lanes;
     return LQT_OK;
 }
 
-lqt_status lqt_cells_get_split(const lqt_cells* c, int* split) {
-    LQT_CHECK(c && split, LQT_ERR_ARG, "lqt_cells_get_split: bad argument");
-    *split = use_split(c) ? 1 : 0;
+lqt_status lqt_cells_get_lanes(const lqt_cells* c, int* lanes) {
+    LQT_CHECK(c && lanes, LQT_ERR_ARG, "lqt_cells_get_lanes: bad argument");
+    *lanes = lanes_per_cell(c);
     return LQT_OK;
 }
 
 lqt_status lqt_cells_step_async(lqt_cells* c, int nsteps, const double* istim, const double* dvgap, void* stream) {
     LQT_CHECK(c && nsteps >= 0, LQT_ERR_ARG, "lqt_cells_step_async: bad argument");
     if (nsteps == 0) return LQT_OK;
-    launch_cells_step(c->fp32, use_split(c), c->n, c->r, c->d, c->pitch, c->v, c->cls, c->pidx, c->consts, nsteps,
-                      c->dt, istim, dvgap, stream ? (hipStream_t)stream : c->stream);
+    launch_cells_step(c->fp32, lanes_per_cell(c), c->n, c->r, c->d, c->pitch, c->v, c->cls, c->pidx, c->consts,
+                      nsteps, c->dt, istim, dvgap, stream ? (hipStream_t)stream : c->stream);
     LQT_HIP(hipGetLastError());
     return LQT_OK;
 }
@@ -517,7 +523,7 @@ lqt_status lqt_cells_pace(lqt_cells* c, const lqt_pacing* pace, double* stats, i
         }
         if (prev_on) a.stim_off[a.nstim - 1] = k;
         a.nsteps = k;
-        launch_cells_pace(c->fp32, use_split(c), c->n, c->r, c->d, c->pitch, c->v, c->cls, c->pidx, c->consts,
+        launch_cells_pace(c->fp32, lanes_per_cell(c), c->n, c->r, c->d, c->pitch, c->v, c->cls, c->pidx, c->consts,
                           c->dt, c->track, a, c->stream);
         LQT_HIP(hipGetLastError());
         time = t;

@@ -81,12 +81,12 @@ struct TissueArgs {
 };
 
 // kernels_cells.hip
-// split: each cell on two lanes of a 128-lane workgroup (same results bit
-// for bit, a shorter step when there are few cells)
-void launch_cells_step(bool fp32, bool split, int n, void* r, double* d, int64_t pitch, void* v,
+// lanes: 1, 2 or 4 lanes per cell (the same results bit for bit; more lanes
+// per cell shorten a step when there are few cells)
+void launch_cells_step(bool fp32, int lanes, int n, void* r, double* d, int64_t pitch, void* v,
                        const uint8_t* cls, const int* pidx, const void* consts, int nsteps, double dt,
                        const double* ist, const double* dvgap, hipStream_t st);
-void launch_cells_pace(bool fp32, bool split, int n, void* r, double* d, int64_t pitch, void* v,
+void launch_cells_pace(bool fp32, int lanes, int n, void* r, double* d, int64_t pitch, void* v,
                        const uint8_t* cls, const int* pidx, const void* consts, double dt, PaceTrack* track,
                        const PaceArgs& a, hipStream_t st);
 

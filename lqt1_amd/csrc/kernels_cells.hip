@@ -242,48 +242,157 @@ __global__ __launch_bounds__(128) void cells_pace_split_kernel(int n, SoA<R> st,
     }
 }
 
-void launch_cells_step(bool fp32, bool split, int n, void* r, double* d, int64_t pitch, void* v,
-                       const uint8_t* cls, const int* pidx, const void* consts, int nsteps, double dt,
-                       const double* ist, const double* dvgap, hipStream_t st) {
-    const int B = split ? 128 : 256;
-    const int G = split ? (n + 63) / 64 : (n + B - 1) / B;
-    if (fp32) {
-        SoA<float> s{(float*)r, d, pitch};
-        auto* cf = (const Consts<float>*)consts;
-        if (split)
-            cells_step_split_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, nsteps, (float)dt, ist,
-                                                            dvgap);
-        else
-            cells_step_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, nsteps, (float)dt, ist, dvgap);
+// ---------------------------------------------------------------------------
+// Four-lane forms: each cell on four lanes of a 256-lane workgroup, one per
+// wavefront, lane l of wavefront q advancing part (q & 1) + 1 of current
+// group A (q < 2) or B (q >= 2) of cell 64 * blockIdx.x + l (tp_group_a /
+// tp_group_b with PART 1 or 2); the four swap their parts' currents through
+// LDS at one barrier per step and finish the step with the same arithmetic,
+// so all four hold the state of the single-lane update bit for bit.  With
+// one such workgroup per compute unit each wavefront has a SIMD of its own.
+// Wavefront 0 writes V, the trackers and the trace.
+
+template <class R>
+struct QuadX {  // one step's currents of the 64 cells of a workgroup
+    R a1[3][64];  // INa, INaL, IKs
+    R a2[2][64];  // Ito, INaK
+    R b1[4][64];  // ICaL, ICaNa, ICaK, Jrel
+    R b2[8][64];  // IKr, IK1, INaCa_i, INaCa_ss, IpCa, ICab, INab, IKb
+};
+
+// One step of a cell on four lanes; X is this step's exchange buffer (the
+// caller alternates two).  Every lane of the workgroup calls it.
+template <class R, bool DIAG>
+__device__ __forceinline__ R quad_update(CellState<R>& s, R v, int q, int l, int c, const Consts<R>& cc, R dt,
+                                         R Ist, R dVgap, QuadX<R>& X, Diag* dg) {
+    Pre<R> P;
+    if (q == 0) {
+        P = tp_pre<R, 1>(s, v, dt);
+        const GroupA<R> g = tp_group_a<R, 1>(s, v, c, cc, dt, P);
+        X.a1[0][l] = g.INa; X.a1[1][l] = g.INaL; X.a1[2][l] = g.IKs;
+    } else if (q == 1) {
+        P = tp_pre<R, 1>(s, v, dt);
+        const GroupA<R> g = tp_group_a<R, 2>(s, v, c, cc, dt, P);
+        X.a2[0][l] = g.Ito; X.a2[1][l] = g.INaK;
+    } else if (q == 2) {
+        P = tp_pre<R, 2>(s, v, dt);
+        const GroupB<R> g = tp_group_b<R, 1>(s, v, c, cc, dt, P);
+        X.b1[0][l] = g.ICaL; X.b1[1][l] = g.ICaNa; X.b1[2][l] = g.ICaK; X.b1[3][l] = g.Jrel;
     } else {
-        SoA<double> s{(double*)r, d, pitch};
-        auto* cd = (const Consts<double>*)consts;
-        if (split)
-            cells_step_split_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, nsteps, dt, ist, dvgap);
-        else
-            cells_step_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, nsteps, dt, ist, dvgap);
+        P = tp_pre<R, 2>(s, v, dt);
+        const GroupB<R> g = tp_group_b<R, 2>(s, v, c, cc, dt, P);
+        X.b2[0][l] = g.IKr; X.b2[1][l] = g.IK1; X.b2[2][l] = g.INaCa_i; X.b2[3][l] = g.INaCa_ss;
+        X.b2[4][l] = g.IpCa; X.b2[5][l] = g.ICab; X.b2[6][l] = g.INab; X.b2[7][l] = g.IKb;
+    }
+    __syncthreads();
+    GroupA<R> GA;
+    GroupB<R> GB;
+    GA.INa = X.a1[0][l]; GA.INaL = X.a1[1][l]; GA.IKs = X.a1[2][l]; GA.Ito = X.a2[0][l]; GA.INaK = X.a2[1][l];
+    GB.ICaL = X.b1[0][l]; GB.ICaNa = X.b1[1][l]; GB.ICaK = X.b1[2][l]; GB.Jrel = X.b1[3][l];
+    GB.IKr = X.b2[0][l]; GB.IK1 = X.b2[1][l]; GB.INaCa_i = X.b2[2][l]; GB.INaCa_ss = X.b2[3][l];
+    GB.IpCa = X.b2[4][l]; GB.ICab = X.b2[5][l]; GB.INab = X.b2[6][l]; GB.IKb = X.b2[7][l];
+    return tp_finish<R, DIAG>(s, v, c, cc, dt, Ist, dVgap, P, GA, GB, dg);
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void cells_step_quad_kernel(int n, SoA<R> st, R* __restrict__ v,
+                                                              const uint8_t* __restrict__ cls,
+                                                              const int* __restrict__ pidx,
+                                                              const Consts<R>* __restrict__ consts, int nsteps,
+                                                              R dt, const double* __restrict__ ist,
+                                                              const double* __restrict__ dvgap) {
+    __shared__ QuadX<R> X[2];
+    const int q = threadIdx.x >> 6;
+    const int l = threadIdx.x & 63;
+    const int k = blockIdx.x * 64 + l;  // the grid holds ceil(n / 64) workgroups
+    const int src = src_cell(k, n);
+    CellState<R> s;
+    load_state(s, st, src);
+    const int c = cls[src];
+    const Consts<R> cc = consts[pidx ? pidx[src] : 0];
+    const R I = ist ? R(ist[src]) : R(0);
+    const R G = dvgap ? R(dvgap[src]) : R(0);
+    R vk = v[src];
+    for (int i = 0; i < nsteps; ++i) vk = quad_update<R, false>(s, vk, q, l, c, cc, dt, I, G, X[i & 1], nullptr);
+    if (k < n) {
+        if (q == 0) v[k] = vk;
+        store_state_quarter(s, st, k, q);
     }
 }
 
-void launch_cells_pace(bool fp32, bool split, int n, void* r, double* d, int64_t pitch, void* v,
-                       const uint8_t* cls, const int* pidx, const void* consts, double dt, PaceTrack* track,
-                       const PaceArgs& a, hipStream_t st) {
-    const int B = split ? 128 : 256;
-    const int G = split ? (n + 63) / 64 : (n + B - 1) / B;
+template <class R>
+__global__ __launch_bounds__(256) void cells_pace_quad_kernel(int n, SoA<R> st, R* __restrict__ v,
+                                                              const uint8_t* __restrict__ cls,
+                                                              const int* __restrict__ pidx,
+                                                              const Consts<R>* __restrict__ consts, R dt,
+                                                              PaceTrack* __restrict__ track, PaceArgs a) {
+    __shared__ QuadX<R> X[2];
+    const int q = threadIdx.x >> 6;
+    const int l = threadIdx.x & 63;
+    const int k = blockIdx.x * 64 + l;  // the grid holds ceil(n / 64) workgroups
+    const int src = src_cell(k, n);
+    CellState<R> s;
+    load_state(s, st, src);
+    const int c = cls[src];
+    const Consts<R> cc = consts[pidx ? pidx[src] : 0];
+    PaceTrack tr = track[src];
+    R vk = v[src];
+    double time = a.time0;
+    PaceState ps{0, a.trace_next, a.trace_slot};
+    for (int i = 0; i < a.nsteps; ++i, time += a.dt) {
+        const R Ist = pace_istim<R>(ps, a, i);
+        Diag dg;
+        vk = quad_update<R, true>(s, vk, q, l, c, cc, dt, Ist, R(0), X[i & 1], &dg);
+        if (q == 0) pace_record(tr, ps, a, k, i, time, (double)vk, dg);  // k >= n is never the trace cell
+    }
+    if (k < n) {
+        if (q == 0) {
+            v[k] = vk;
+            track[k] = tr;
+        }
+        store_state_quarter(s, st, k, q);
+    }
+}
+
+void launch_cells_step(bool fp32, int lanes, int n, void* r, double* d, int64_t pitch, void* v,
+                       const uint8_t* cls, const int* pidx, const void* consts, int nsteps, double dt,
+                       const double* ist, const double* dvgap, hipStream_t st) {
+    const int B = lanes == 4 ? 256 : lanes == 2 ? 128 : 256;
+    const int G = lanes == 1 ? (n + B - 1) / B : (n + 63) / 64;
     if (fp32) {
         SoA<float> s{(float*)r, d, pitch};
         auto* cf = (const Consts<float>*)consts;
-        if (split)
-            cells_pace_split_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, (float)dt, track, a);
-        else
-            cells_pace_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, (float)dt, track, a);
+        const float dtf = (float)dt;
+        if (lanes == 4) cells_step_quad_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, nsteps, dtf, ist, dvgap);
+        else if (lanes == 2) cells_step_split_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, nsteps, dtf, ist, dvgap);
+        else cells_step_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, nsteps, dtf, ist, dvgap);
     } else {
         SoA<double> s{(double*)r, d, pitch};
         auto* cd = (const Consts<double>*)consts;
-        if (split)
-            cells_pace_split_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, dt, track, a);
-        else
-            cells_pace_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, dt, track, a);
+        if (lanes == 4) cells_step_quad_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, nsteps, dt, ist, dvgap);
+        else if (lanes == 2) cells_step_split_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, nsteps, dt, ist, dvgap);
+        else cells_step_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, nsteps, dt, ist, dvgap);
+    }
+}
+
+void launch_cells_pace(bool fp32, int lanes, int n, void* r, double* d, int64_t pitch, void* v,
+                       const uint8_t* cls, const int* pidx, const void* consts, double dt, PaceTrack* track,
+                       const PaceArgs& a, hipStream_t st) {
+    const int B = lanes == 4 ? 256 : lanes == 2 ? 128 : 256;
+    const int G = lanes == 1 ? (n + B - 1) / B : (n + 63) / 64;
+    if (fp32) {
+        SoA<float> s{(float*)r, d, pitch};
+        auto* cf = (const Consts<float>*)consts;
+        const float dtf = (float)dt;
+        if (lanes == 4) cells_pace_quad_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, dtf, track, a);
+        else if (lanes == 2) cells_pace_split_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, dtf, track, a);
+        else cells_pace_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, dtf, track, a);
+    } else {
+        SoA<double> s{(double*)r, d, pitch};
+        auto* cd = (const Consts<double>*)consts;
+        if (lanes == 4) cells_pace_quad_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, dt, track, a);
+        else if (lanes == 2) cells_pace_split_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, dt, track, a);
+        else cells_pace_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, dt, track, a);
     }
 }
 

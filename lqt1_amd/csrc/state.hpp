@@ -110,6 +110,33 @@ __device__ __forceinline__ void store_state_half(const CellState<R>& s, const So
     }
 }
 
+// The quarter of a cell split over four lanes that advances gate slot
+// `slot`: 0 the gates of tp_group_a part 1 (INa, INaL, IKs), 1 those of part
+// 2 (Ito), 2 those of tp_group_b part 1 (ICaL, Jrel), 3 those of part 2 (IKr,
+// IK1).  Every other slot (the concentrations) belongs to quarter 0.
+__host__ __device__ constexpr int quarter_of(int slot) {
+    return (slot == RS_a || slot == RS_iF || slot == RS_iS || slot == RS_ap || slot == RS_iFp || slot == RS_iSp) ? 1
+           : (slot == RS_xrf || slot == RS_xrs || slot == RS_xk1)                                              ? 3
+           : b_gate(slot)                                                                                       ? 2
+                                                                                                                : 0;
+}
+
+// store_state for one lane of a cell split over four lanes: quarter q writes
+// the slots it owns.
+template <class R>
+__device__ __forceinline__ void store_state_quarter(const CellState<R>& s, const SoA<R>& a, uint32_t k, int q) {
+    const uint32_t orr = k * (uint32_t)sizeof(R), od = k * 8u;
+#define LQT_ST(n) \
+    if (quarter_of(RS_##n) == q) st_nt(s.n, a.r + (size_t)RS_##n * a.pitch, orr);
+    LQT_RFIELDS(LQT_ST)
+#undef LQT_ST
+    if (q == 0) {
+#define LQT_ST(n) st_nt(s.n, a.d + (size_t)DS_##n * a.pitch, od);
+        LQT_DFIELDS(LQT_ST)
+#undef LQT_ST
+    }
+}
+
 // Chunk-major storage (the tissue's state and static voxel data): voxel k
 // belongs to chunk k / 64, the voxels of one wavefront, and a chunk keeps
 // its slots one after another, 64 values each.  A wavefront then reads and

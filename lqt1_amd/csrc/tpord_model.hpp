@@ -266,7 +266,9 @@ __device__ __forceinline__ Pre<R> tp_pre(const CellState<R>& s, R v, R dt) {
                   e1, e2, r1, r2};
 }
 
-template <class R>
+// PART: 0 the whole group; 1 INa, INaL and IKs with their gates; 2 Ito and
+// INaK with Ito's gates (the currents of the other part are returned as 0).
+template <class R, int PART = 0>
 __device__ __forceinline__ GroupA<R> tp_group_a(CellState<R>& s, R v, int cls, const Consts<R>& c, R dt,
                                                 const Pre<R>& P) {
     using M = Math<R>;
@@ -279,52 +281,54 @@ __device__ __forceinline__ GroupA<R> tp_group_a(CellState<R>& s, R v, int cls, c
     (void)epi; (void)mid; (void)nai; (void)ki; (void)cansr; (void)cajsr; (void)CaMKt; (void)nass; (void)kss;
     (void)cai; (void)cass; (void)ENa; (void)EK; (void)EKs; (void)CaMKb; (void)fCaMKp; (void)vfrt; (void)ndt;
     (void)e1; (void)e2; (void)r1; (void)r2;
+    R INa = R(0), INaL = R(0), Ito = R(0), IKs = R(0), INaK = R(0);
 
-    // ---- TP06 INa (:465-504)
-    {
-        const R AM = M::rcp(R(1) + M::exl(v, -1.0 / 5.0, -60.0 / 5.0));
-        const R BM = R(0.1) * M::rcp(R(1) + M::exl(v, 1.0 / 5.0, 35.0 / 5.0)) +
-                     R(0.10) * M::rcp(R(1) + M::exl(v, 1.0 / 200.0, -50.0 / 200.0));
-        const R TAU_M = AM * BM;
-        const R em = R(1) + M::exl(v, -1.0 / 9.03, -56.86 / 9.03);
-        const R M_INF = M::rcp(em * em);
-        R rate_h, rate_j;  // 1/TAU_H, 1/TAU_J
-        if (v >= R(-40)) {
-            rate_h = R(0.77) * M::rcp(R(0.13) * (R(1) + M::exl(v, -1.0 / 11.1, -10.66 / 11.1)));
-            rate_j = R(0.6) * M::exl(v, 0.057, 0.0) * M::rcp(R(1) + M::exl(v, -0.1, -3.2));
-        } else {
-            rate_h = R(0.057) * M::exl(v, -1.0 / 6.8, -80.0 / 6.8) +
-                     (R(2.7) * M::exl(v, 0.079, 0.0) + R(3.1e5) * M::exl(v, 0.3485, 0.0));
-            const R AJ = ((R(-2.5428e4) * M::exl(v, 0.2444, 0.0) - R(6.948e-6) * M::exl(v, -0.04391, 0.0)) *
-                          (v + R(37.78))) *
-                         M::rcp(R(1) + M::exl(v, 0.311, 0.311 * 79.23));
-            const R BJ = R(0.02424) * M::exl(v, -0.01052, 0.0) * M::rcp(R(1) + M::exl(v, -0.1378, -0.1378 * 40.14));
-            rate_j = AJ + BJ;
+    if constexpr (PART != 2) {
+        // ---- TP06 INa (:465-504)
+        {
+            const R AM = M::rcp(R(1) + M::exl(v, -1.0 / 5.0, -60.0 / 5.0));
+            const R BM = R(0.1) * M::rcp(R(1) + M::exl(v, 1.0 / 5.0, 35.0 / 5.0)) +
+                         R(0.10) * M::rcp(R(1) + M::exl(v, 1.0 / 200.0, -50.0 / 200.0));
+            const R TAU_M = AM * BM;
+            const R em = R(1) + M::exl(v, -1.0 / 9.03, -56.86 / 9.03);
+            const R M_INF = M::rcp(em * em);
+            R rate_h, rate_j;  // 1/TAU_H, 1/TAU_J
+            if (v >= R(-40)) {
+                rate_h = R(0.77) * M::rcp(R(0.13) * (R(1) + M::exl(v, -1.0 / 11.1, -10.66 / 11.1)));
+                rate_j = R(0.6) * M::exl(v, 0.057, 0.0) * M::rcp(R(1) + M::exl(v, -0.1, -3.2));
+            } else {
+                rate_h = R(0.057) * M::exl(v, -1.0 / 6.8, -80.0 / 6.8) +
+                         (R(2.7) * M::exl(v, 0.079, 0.0) + R(3.1e5) * M::exl(v, 0.3485, 0.0));
+                const R AJ = ((R(-2.5428e4) * M::exl(v, 0.2444, 0.0) - R(6.948e-6) * M::exl(v, -0.04391, 0.0)) *
+                              (v + R(37.78))) *
+                             M::rcp(R(1) + M::exl(v, 0.311, 0.311 * 79.23));
+                const R BJ = R(0.02424) * M::exl(v, -0.01052, 0.0) * M::rcp(R(1) + M::exl(v, -0.1378, -0.1378 * 40.14));
+                rate_j = AJ + BJ;
+            }
+            const R eh = R(1) + M::exl(v, 1.0 / 7.43, 71.55 / 7.43);
+            const R H_INF = M::rcp(eh * eh);
+            s.sm = M_INF - (M_INF - s.sm) * M::decay(ndt, M::rcp(TAU_M));
+            s.sh = H_INF - (H_INF - s.sh) * M::decay(ndt, rate_h);
+            s.sj = H_INF - (H_INF - s.sj) * M::decay(ndt, rate_j);
         }
-        const R eh = R(1) + M::exl(v, 1.0 / 7.43, 71.55 / 7.43);
-        const R H_INF = M::rcp(eh * eh);
-        s.sm = M_INF - (M_INF - s.sm) * M::decay(ndt, M::rcp(TAU_M));
-        s.sh = H_INF - (H_INF - s.sh) * M::decay(ndt, rate_h);
-        s.sj = H_INF - (H_INF - s.sj) * M::decay(ndt, rate_j);
-    }
-    const R INa = R(14.838) * s.sm * s.sm * s.sm * s.sh * s.sj * (v - ENa);
+        INa = R(14.838) * s.sm * s.sm * s.sm * s.sh * s.sj * (v - ENa);
 
-    // ---- INaL (:509-528)
-    {
-        const R mLss = M::rcp(R(1) + M::exl(v, -1.0 / 5.264, -42.85 / 5.264));
-        const R tm_rate = R(6.765) * M::exl(v, 1.0 / 34.77, 11.64 / 34.77) + R(8.552) * M::exl(v, -1.0 / 5.955, -77.42 / 5.955);
-        s.mL = mLss - (mLss - s.mL) * M::decay(ndt, tm_rate);
-        const R hLss = M::rcp(R(1) + M::exl(v, 1.0 / 7.488, 87.61 / 7.488));
-        s.hL = M::rl_slow(s.hL, hLss, dt, R(1.0 / 200.0));
-        const R hLssp = M::rcp(R(1) + M::exl(v, 1.0 / 7.488, 93.81 / 7.488));
-        s.hLp = M::rl_slow(s.hLp, hLssp, dt, R(1.0 / 600.0));
+        // ---- INaL (:509-528)
+        {
+            const R mLss = M::rcp(R(1) + M::exl(v, -1.0 / 5.264, -42.85 / 5.264));
+            const R tm_rate = R(6.765) * M::exl(v, 1.0 / 34.77, 11.64 / 34.77) + R(8.552) * M::exl(v, -1.0 / 5.955, -77.42 / 5.955);
+            s.mL = mLss - (mLss - s.mL) * M::decay(ndt, tm_rate);
+            const R hLss = M::rcp(R(1) + M::exl(v, 1.0 / 7.488, 87.61 / 7.488));
+            s.hL = M::rl_slow(s.hL, hLss, dt, R(1.0 / 200.0));
+            const R hLssp = M::rcp(R(1) + M::exl(v, 1.0 / 7.488, 93.81 / 7.488));
+            s.hLp = M::rl_slow(s.hLp, hLssp, dt, R(1.0 / 600.0));
+        }
+        const R GNaL = epi ? R(0.0075 * 0.6) : R(0.0075);
+        INaL = GNaL * (v - ENa) * s.mL * ((R(1) - fCaMKp) * s.hL + fCaMKp * s.hLp);
     }
-    const R GNaL = epi ? R(0.0075 * 0.6) : R(0.0075);
-    const R INaL = GNaL * (v - ENa) * s.mL * ((R(1) - fCaMKp) * s.hL + fCaMKp * s.hLp);
 
     // ---- Ito (:530-571)
-    R Ito;
-    {
+    if constexpr (PART != 1) {
         const R ass = M::rcp(R(1) + M::exl(v, -1.0 / 14.82, 14.34 / 14.82));
         const R ta = R(1.0515) * M::rcp(M::rcp(R(1.2089) * (R(1) + M::exl(v, -1.0 / 29.3814, 18.4099 / 29.3814))) +
                                         R(3.5) * M::rcp(R(1) + M::exl(v, 1.0 / 29.3814, 100.0 / 29.3814)));
@@ -359,8 +363,7 @@ __device__ __forceinline__ GroupA<R> tp_group_a(CellState<R>& s, R v, int cls, c
     }
 
     // ---- IKs (:680-735)
-    R IKs;
-    {
+    if constexpr (PART != 2) {
         const R xs1ss = M::rcp(R(1) + M::exl(v, -1.0 / 8.932, -11.60 / 8.932));
         const R txs1 = R(817.3) + M::rcp(R(2.326e-4) * M::exl(v, 1.0 / 17.80, 48.28 / 17.80) +
                                          R(0.001292) * M::exl(v, -1.0 / 230.0, -210.0 / 230.0));
@@ -372,8 +375,7 @@ __device__ __forceinline__ GroupA<R> tp_group_a(CellState<R>& s, R v, int cls, c
     }
 
     // ---- INaK (:854-906)
-    R INaK;
-    {
+    if constexpr (PART != 1) {
         const R Knai = R(9.073) * M::exl(v, -0.1550 * k::FRT / 3.0, 0.0);
         const R Knao = R(27.78) * M::exl(v, (1.0 + 0.1550) * k::FRT / 3.0, 0.0);
         const R P = R(k::eP) * M::rcp(R(1.0 + k::H / k::Khp) + nai * R(1.0 / k::Knap) + ki * R(1.0 / k::Kxkur));
@@ -407,7 +409,10 @@ __device__ __forceinline__ GroupA<R> tp_group_a(CellState<R>& s, R v, int cls, c
     return GroupA<R>{INa, INaL, Ito, IKs, INaK};
 }
 
-template <class R>
+// PART: 0 the whole group; 1 ICaL/ICaNa/ICaK and Jrel (whose steady state
+// follows ICaL) with their gates; 2 IKr, IK1, INaCa, IpCa and the background
+// currents with the gates of IKr and IK1.
+template <class R, int PART = 0>
 __device__ __forceinline__ GroupB<R> tp_group_b(CellState<R>& s, R v, int cls, const Consts<R>& c, R dt,
                                                 const Pre<R>& P) {
     using M = Math<R>;
@@ -420,10 +425,11 @@ __device__ __forceinline__ GroupB<R> tp_group_b(CellState<R>& s, R v, int cls, c
     (void)epi; (void)mid; (void)nai; (void)ki; (void)cansr; (void)cajsr; (void)CaMKt; (void)nass; (void)kss;
     (void)cai; (void)cass; (void)ENa; (void)EK; (void)EKs; (void)CaMKb; (void)fCaMKp; (void)vfrt; (void)ndt;
     (void)e1; (void)e2; (void)r1; (void)r2;
+    R ICaL = R(0), ICaNa = R(0), ICaK = R(0), IKr = R(0), IK1 = R(0), INaCa_i = R(0), INaCa_ss = R(0);
+    R IpCa = R(0), ICab = R(0), Jrel = R(0), INab = R(0), IKb = R(0);
 
     // ---- ICaL, ICaNa, ICaK (:573-652)
-    R ICaL, ICaNa, ICaK;
-    {
+    if constexpr (PART != 2) {
         const R vs = v + c.dss_shift;
         const R dss = M::rcp(R(1) + M::exl(vs, -1.0 / 4.230, -3.940 / 4.230));
         const R td = R(0.6) + M::rcp(M::exl(v, -0.05, -0.3) + M::exl(v, 0.09, 1.26));
@@ -471,73 +477,75 @@ __device__ __forceinline__ GroupB<R> tp_group_b(CellState<R>& s, R v, int cls, c
         ICaK = R(3.574e-4) * PCa * PhiCaK * mix;
     }
 
-    // ---- IKr (:654-678)
-    R IKr;
-    {
-        const R xrss = M::rcp(R(1) + M::exl(v, -1.0 / 6.789, -8.337 / 6.789));
-        const R txrf = R(12.98) + M::rcp(R(0.3652) * M::exl(v, 1.0 / 3.869, -31.66 / 3.869) +
-                                         R(4.123e-5) * M::exl(v, -1.0 / 20.38, 47.78 / 20.38));
-        const R txrs = R(1.865) + M::rcp(R(0.06629) * M::exl(v, 1.0 / 7.355, -34.70 / 7.355) +
-                                         R(1.128e-5) * M::exl(v, -1.0 / 25.94, 29.74 / 25.94));
-        const R Axrf = M::rcp(R(1) + M::exl(v, 1.0 / 38.21, 54.81 / 38.21));
-        s.xrf = M::rl_slow(s.xrf, xrss, dt, M::rcp(txrf));
-        s.xrs = M::rl_slow(s.xrs, xrss, dt, M::rcp(txrs));
-        const R xr = Axrf * s.xrf + (R(1) - Axrf) * s.xrs;
-        const R rkr = M::rcp((R(1) + M::exl(v, 1.0 / 75.0, 55.0 / 75.0)) * (R(1) + M::exl(v, 1.0 / 30.0, -10.0 / 30.0)));
-        const R GKr = epi ? R(0.046 * 1.3) : (mid ? R(0.046 * 0.8) : R(0.046));
-        IKr = GKr * xr * rkr * (v - EK);  // sqrt(ko/5.4) == 1
+    if constexpr (PART != 1) {
+        // ---- IKr (:654-678)
+        {
+            const R xrss = M::rcp(R(1) + M::exl(v, -1.0 / 6.789, -8.337 / 6.789));
+            const R txrf = R(12.98) + M::rcp(R(0.3652) * M::exl(v, 1.0 / 3.869, -31.66 / 3.869) +
+                                             R(4.123e-5) * M::exl(v, -1.0 / 20.38, 47.78 / 20.38));
+            const R txrs = R(1.865) + M::rcp(R(0.06629) * M::exl(v, 1.0 / 7.355, -34.70 / 7.355) +
+                                             R(1.128e-5) * M::exl(v, -1.0 / 25.94, 29.74 / 25.94));
+            const R Axrf = M::rcp(R(1) + M::exl(v, 1.0 / 38.21, 54.81 / 38.21));
+            s.xrf = M::rl_slow(s.xrf, xrss, dt, M::rcp(txrf));
+            s.xrs = M::rl_slow(s.xrs, xrss, dt, M::rcp(txrs));
+            const R xr = Axrf * s.xrf + (R(1) - Axrf) * s.xrs;
+            const R rkr = M::rcp((R(1) + M::exl(v, 1.0 / 75.0, 55.0 / 75.0)) * (R(1) + M::exl(v, 1.0 / 30.0, -10.0 / 30.0)));
+            const R GKr = epi ? R(0.046 * 1.3) : (mid ? R(0.046 * 0.8) : R(0.046));
+            IKr = GKr * xr * rkr * (v - EK);  // sqrt(ko/5.4) == 1
+        }
+
+        // ---- IK1 (:738-751)
+        {
+            const R xk1ss = M::rcp(R(1) + M::exl(v, -1.0 / (1.5692 * k::ko + 3.8115), -(2.5538 * k::ko + 144.59) / (1.5692 * k::ko + 3.8115)));
+            const R rate_k1 = (M::exl(v, -1.0 / 20.36, -127.2 / 20.36) + M::exl(v, 1.0 / 69.33, 236.8 / 69.33)) * R(1.0 / 122.2);
+            s.xk1 = xk1ss - (xk1ss - s.xk1) * M::decay(ndt, rate_k1);
+            const R rk1 = M::rcp(R(1) + M::exl(v, 1.0 / 9.493, (105.8 - 2.6 * k::ko) / 9.493));
+            const R GK1 = epi ? R(0.1908 * 1.2) : (mid ? R(0.1908 * 1.3) : R(0.1908));
+            IK1 = GK1 * M::sqrtc(k::ko) * rk1 * s.xk1 * (v - EK);
+        }
+
+        // ---- INaCa (:753-852)
+        {
+            const R hca = M::exl(v, 0.1670 * k::FRT, 0.0);
+            const R hna = M::exl(v, 0.5224 * k::FRT, 0.0);
+            const R inv_hna = M::rcp(hna);
+            const R inv_hca = M::rcp(hca);
+            const R Gncx = epi ? R(0.0008 * 1.1) : (mid ? R(0.0008 * 1.4) : R(0.0008));
+            const R ri = R(150.0e-6) * M::rcp(cai);
+            const R allo_i = M::rcp(R(1) + ri * ri);
+            INaCa_i = R(0.8) * Gncx * allo_i * ncx<R>(nai, cai, hna, inv_hna, inv_hca);
+            const R rs = R(150.0e-6) * M::rcp(cass);
+            const R allo_ss = M::rcp(R(1) + rs * rs);
+            INaCa_ss = R(0.2) * Gncx * allo_ss * ncx<R>(nass, cass, hna, inv_hna, inv_hca);
+        }
+
+        // background (:908-923)
+        const R xkb = M::rcp(R(1) + M::exl(v, -1.0 / 18.34, 14.48 / 18.34));
+        const R GKb = epi ? R(0.003 * 0.6) : R(0.003);
+        IKb = GKb * xkb * (v - EK);
+        INab = R(3.75e-10 * k::F) * r1 * (nai * e1 - R(k::nao));
+        ICab = R(2.5e-8 * 2.0 * k::F) * r2 * (cai * e2 - R(0.341 * k::cao));
+        IpCa = R(0.0005) * cai * M::rcp(R(0.0005) + cai);
     }
 
-    // ---- IK1 (:738-751)
-    R IK1;
-    {
-        const R xk1ss = M::rcp(R(1) + M::exl(v, -1.0 / (1.5692 * k::ko + 3.8115), -(2.5538 * k::ko + 144.59) / (1.5692 * k::ko + 3.8115)));
-        const R rate_k1 = (M::exl(v, -1.0 / 20.36, -127.2 / 20.36) + M::exl(v, 1.0 / 69.33, 236.8 / 69.33)) * R(1.0 / 122.2);
-        s.xk1 = xk1ss - (xk1ss - s.xk1) * M::decay(ndt, rate_k1);
-        const R rk1 = M::rcp(R(1) + M::exl(v, 1.0 / 9.493, (105.8 - 2.6 * k::ko) / 9.493));
-        const R GK1 = epi ? R(0.1908 * 1.2) : (mid ? R(0.1908 * 1.3) : R(0.1908));
-        IK1 = GK1 * M::sqrtc(k::ko) * rk1 * s.xk1 * (v - EK);
+    if constexpr (PART != 2) {
+        // Jrel (:946-979), RL gates Jrelnp/Jrelp driven by ICaL
+        const R rj = R(1.5) * M::rcp(cajsr);
+        const R rj2 = rj * rj;
+        const R rj4 = rj2 * rj2;
+        const R jrel_den = M::rcp(R(1) + rj4 * rj4);
+        const R mfac = mid ? R(1.7) : R(1);
+        const R Jrel_inf = R(0.5 * 4.75) * (-ICaL) * jrel_den * mfac;
+        const R q_rel = M::rcp(R(1) + R(0.0123) * M::rcp(cajsr));
+        R tau_rel = R(4.75) * q_rel;
+        tau_rel = tau_rel < R(0.005) ? R(0.005) : tau_rel;
+        s.Jrelnp = Jrel_inf - (Jrel_inf - s.Jrelnp) * M::decay(ndt, M::rcp(tau_rel));
+        const R Jrel_infp = R(0.5 * 1.25 * 4.75) * (-ICaL) * jrel_den * mfac;
+        R tau_relp = R(1.25 * 4.75) * q_rel;
+        tau_relp = tau_relp < R(0.005) ? R(0.005) : tau_relp;
+        s.Jrelp = Jrel_infp - (Jrel_infp - s.Jrelp) * M::decay(ndt, M::rcp(tau_relp));
+        Jrel = (R(1) - fCaMKp) * s.Jrelnp + fCaMKp * s.Jrelp;
     }
-
-    // ---- INaCa (:753-852)
-    R INaCa_i, INaCa_ss;
-    {
-        const R hca = M::exl(v, 0.1670 * k::FRT, 0.0);
-        const R hna = M::exl(v, 0.5224 * k::FRT, 0.0);
-        const R inv_hna = M::rcp(hna);
-        const R inv_hca = M::rcp(hca);
-        const R Gncx = epi ? R(0.0008 * 1.1) : (mid ? R(0.0008 * 1.4) : R(0.0008));
-        const R ri = R(150.0e-6) * M::rcp(cai);
-        const R allo_i = M::rcp(R(1) + ri * ri);
-        INaCa_i = R(0.8) * Gncx * allo_i * ncx<R>(nai, cai, hna, inv_hna, inv_hca);
-        const R rs = R(150.0e-6) * M::rcp(cass);
-        const R allo_ss = M::rcp(R(1) + rs * rs);
-        INaCa_ss = R(0.2) * Gncx * allo_ss * ncx<R>(nass, cass, hna, inv_hna, inv_hca);
-    }
-
-    // background (:908-923)
-    const R xkb = M::rcp(R(1) + M::exl(v, -1.0 / 18.34, 14.48 / 18.34));
-    const R GKb = epi ? R(0.003 * 0.6) : R(0.003);
-    const R IKb = GKb * xkb * (v - EK);
-    const R INab = R(3.75e-10 * k::F) * r1 * (nai * e1 - R(k::nao));
-    const R ICab = R(2.5e-8 * 2.0 * k::F) * r2 * (cai * e2 - R(0.341 * k::cao));
-    const R IpCa = R(0.0005) * cai * M::rcp(R(0.0005) + cai);
-    // Jrel (:946-979), RL gates Jrelnp/Jrelp driven by ICaL
-    const R rj = R(1.5) * M::rcp(cajsr);
-    const R rj2 = rj * rj;
-    const R rj4 = rj2 * rj2;
-    const R jrel_den = M::rcp(R(1) + rj4 * rj4);
-    const R mfac = mid ? R(1.7) : R(1);
-    const R Jrel_inf = R(0.5 * 4.75) * (-ICaL) * jrel_den * mfac;
-    const R q_rel = M::rcp(R(1) + R(0.0123) * M::rcp(cajsr));
-    R tau_rel = R(4.75) * q_rel;
-    tau_rel = tau_rel < R(0.005) ? R(0.005) : tau_rel;
-    s.Jrelnp = Jrel_inf - (Jrel_inf - s.Jrelnp) * M::decay(ndt, M::rcp(tau_rel));
-    const R Jrel_infp = R(0.5 * 1.25 * 4.75) * (-ICaL) * jrel_den * mfac;
-    R tau_relp = R(1.25 * 4.75) * q_rel;
-    tau_relp = tau_relp < R(0.005) ? R(0.005) : tau_relp;
-    s.Jrelp = Jrel_infp - (Jrel_infp - s.Jrelp) * M::decay(ndt, M::rcp(tau_relp));
-    const R Jrel = (R(1) - fCaMKp) * s.Jrelnp + fCaMKp * s.Jrelp;
     return GroupB<R>{ICaL, ICaNa, ICaK, IKr, IK1, INaCa_i, INaCa_ss, IpCa, ICab, Jrel, INab, IKb};
 }
 
