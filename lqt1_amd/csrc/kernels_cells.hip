@@ -1,13 +1,14 @@
-// Independent-cell kernels: one TPORd cell per lane (or, split, per two
+// Independent-cell kernels: one TPORd cell per lane (or per two or four
 // lanes), state kept in registers across the steps of one launch.
 //
 // Every lane of a wavefront that holds at least one cell computes: lanes
-// past the last cell run a copy of one of the wavefront's own cells (reading
-// its inputs before the wavefront writes anything, writing nothing).  A wavefront whose live lanes all lie in one 16-lane quarter
+// past the last cell run a copy of one of the wavefront's own cells
+// (reading its inputs before the wavefront writes anything, writing
+// nothing).  A wavefront whose live lanes all lie in one 16-lane quarter
 // steps about 1.3x slower than a full one on gfx950 (measured with the
-// pacing protocol: 9.2 against 6.9 us per FP64 step split, 2.55 against
-// 1.95 us FP32 split, 11.3 against 10.2 us FP64 on one lane, for 1-16
-// cells against 64 or more); the idle lanes cost nothing to fill.
+// pacing protocol: 9.2 against 6.9 us per FP64 step on two lanes per cell,
+// 2.55 against 1.95 us FP32, 11.3 against 10.2 us FP64 on one lane, for
+// 1-16 cells against 64 or more); the idle lanes cost nothing to fill.
 #include "internal.hpp"
 
 namespace lqt {
@@ -354,46 +355,53 @@ __global__ __launch_bounds__(256) void cells_pace_quad_kernel(int n, SoA<R> st, 
     }
 }
 
+// Grid and block of each layout: one lane per cell in 256-lane workgroups;
+// two or four lanes per cell in workgroups of 64 cells.
+template <class R>
+static void launch_step_r(int lanes, int n, SoA<R> s, R* v, const uint8_t* cls, const int* pidx,
+                          const Consts<R>* consts, int nsteps, R dt, const double* ist, const double* dvgap,
+                          hipStream_t st) {
+    const int groups = (n + 63) / 64;
+    if (lanes == 4)
+        cells_step_quad_kernel<R><<<groups, 256, 0, st>>>(n, s, v, cls, pidx, consts, nsteps, dt, ist, dvgap);
+    else if (lanes == 2)
+        cells_step_split_kernel<R><<<groups, 128, 0, st>>>(n, s, v, cls, pidx, consts, nsteps, dt, ist, dvgap);
+    else
+        cells_step_kernel<R><<<(n + 255) / 256, 256, 0, st>>>(n, s, v, cls, pidx, consts, nsteps, dt, ist, dvgap);
+}
+
+template <class R>
+static void launch_pace_r(int lanes, int n, SoA<R> s, R* v, const uint8_t* cls, const int* pidx,
+                          const Consts<R>* consts, R dt, PaceTrack* track, const PaceArgs& a, hipStream_t st) {
+    const int groups = (n + 63) / 64;
+    if (lanes == 4)
+        cells_pace_quad_kernel<R><<<groups, 256, 0, st>>>(n, s, v, cls, pidx, consts, dt, track, a);
+    else if (lanes == 2)
+        cells_pace_split_kernel<R><<<groups, 128, 0, st>>>(n, s, v, cls, pidx, consts, dt, track, a);
+    else
+        cells_pace_kernel<R><<<(n + 255) / 256, 256, 0, st>>>(n, s, v, cls, pidx, consts, dt, track, a);
+}
+
 void launch_cells_step(bool fp32, int lanes, int n, void* r, double* d, int64_t pitch, void* v,
                        const uint8_t* cls, const int* pidx, const void* consts, int nsteps, double dt,
                        const double* ist, const double* dvgap, hipStream_t st) {
-    const int B = lanes == 4 ? 256 : lanes == 2 ? 128 : 256;
-    const int G = lanes == 1 ? (n + B - 1) / B : (n + 63) / 64;
-    if (fp32) {
-        SoA<float> s{(float*)r, d, pitch};
-        auto* cf = (const Consts<float>*)consts;
-        const float dtf = (float)dt;
-        if (lanes == 4) cells_step_quad_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, nsteps, dtf, ist, dvgap);
-        else if (lanes == 2) cells_step_split_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, nsteps, dtf, ist, dvgap);
-        else cells_step_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, nsteps, dtf, ist, dvgap);
-    } else {
-        SoA<double> s{(double*)r, d, pitch};
-        auto* cd = (const Consts<double>*)consts;
-        if (lanes == 4) cells_step_quad_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, nsteps, dt, ist, dvgap);
-        else if (lanes == 2) cells_step_split_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, nsteps, dt, ist, dvgap);
-        else cells_step_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, nsteps, dt, ist, dvgap);
-    }
+    if (fp32)
+        launch_step_r<float>(lanes, n, SoA<float>{(float*)r, d, pitch}, (float*)v, cls, pidx,
+                             (const Consts<float>*)consts, nsteps, (float)dt, ist, dvgap, st);
+    else
+        launch_step_r<double>(lanes, n, SoA<double>{(double*)r, d, pitch}, (double*)v, cls, pidx,
+                              (const Consts<double>*)consts, nsteps, dt, ist, dvgap, st);
 }
 
 void launch_cells_pace(bool fp32, int lanes, int n, void* r, double* d, int64_t pitch, void* v,
                        const uint8_t* cls, const int* pidx, const void* consts, double dt, PaceTrack* track,
                        const PaceArgs& a, hipStream_t st) {
-    const int B = lanes == 4 ? 256 : lanes == 2 ? 128 : 256;
-    const int G = lanes == 1 ? (n + B - 1) / B : (n + 63) / 64;
-    if (fp32) {
-        SoA<float> s{(float*)r, d, pitch};
-        auto* cf = (const Consts<float>*)consts;
-        const float dtf = (float)dt;
-        if (lanes == 4) cells_pace_quad_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, dtf, track, a);
-        else if (lanes == 2) cells_pace_split_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, dtf, track, a);
-        else cells_pace_kernel<float><<<G, B, 0, st>>>(n, s, (float*)v, cls, pidx, cf, dtf, track, a);
-    } else {
-        SoA<double> s{(double*)r, d, pitch};
-        auto* cd = (const Consts<double>*)consts;
-        if (lanes == 4) cells_pace_quad_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, dt, track, a);
-        else if (lanes == 2) cells_pace_split_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, dt, track, a);
-        else cells_pace_kernel<double><<<G, B, 0, st>>>(n, s, (double*)v, cls, pidx, cd, dt, track, a);
-    }
+    if (fp32)
+        launch_pace_r<float>(lanes, n, SoA<float>{(float*)r, d, pitch}, (float*)v, cls, pidx,
+                             (const Consts<float>*)consts, (float)dt, track, a, st);
+    else
+        launch_pace_r<double>(lanes, n, SoA<double>{(double*)r, d, pitch}, (double*)v, cls, pidx,
+                              (const Consts<double>*)consts, dt, track, a, st);
 }
 
 }  // namespace lqt
